@@ -1,0 +1,183 @@
+"""le2d_amd -- MI355X split-step elasticity solver behind the le_core.h C ABI.
+
+The product is the pair of shared libraries built from le2d_amd/csrc
+(lib/liblecore_d.so with real = double, lib/liblecore_f.so with real =
+float).  This module is a thin ctypes front end over that ABI for Python
+callers, tests and the benchmark; every numeric operation runs in the HIP
+library.  There is no CPU fallback: loading fails if the library was not
+built, and stepping aborts inside the library if no GPU is present.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .abi import ABI_D, ABI_F, ST_AOS, ST_SOA, NODE_FIELDS  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+
+SYNC_WAIT, SYNC_ASYNC, SYNC_COHERENT = 0, 1, 2
+
+# Symbols declared in include/le2d.h (extensions), beyond simple/le_core.h.
+_EXT = {
+    "le2d_last_error": ([], ctypes.c_char_p),
+    "le2d_set_sync_mode": (["T", ctypes.c_int], ctypes.c_int),
+    "le2d_wait": (["T"], ctypes.c_int),
+    "le_sync_host": (["T"], ctypes.c_int),
+    "le_mark_host_dirty": (["T"], ctypes.c_int),
+    "le_step_n": (["T", ctypes.c_int], ctypes.c_int),
+    "le2d_set_tiles": (["T", ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "le2d_get_tiles": (["T", ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                        ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "le2d_time_sweeps": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "le2d_device_bytes": (["T"], ctypes.c_size_t),
+}
+
+_libs = {}
+
+
+def lib_path(dtype=np.float64):
+    name = "liblecore_d.so" if np.dtype(dtype) == np.float64 else "liblecore_f.so"
+    return os.path.join(LIBDIR, name)
+
+
+def load(dtype=np.float64):
+    """Load (once) the C-ABI library for `dtype` and bind its entry points."""
+    dtype = np.dtype(dtype)
+    if dtype in _libs:
+        return _libs[dtype]
+    path = lib_path(dtype)
+    if not os.path.exists(path):
+        raise RuntimeError(f"le2d_amd: {path} is not built; run __graft_entry__.build() "
+                           "(make -C le2d_amd/csrc)")
+    lib = ctypes.CDLL(path)
+    abi = ABI_D if dtype == np.float64 else ABI_F
+    abi.bind(lib)
+    PT = ctypes.POINTER(abi.le_task)
+    for name, (args, res) in _EXT.items():
+        fn = getattr(lib, name)
+        fn.argtypes = [PT if a == "T" else a for a in args]
+        fn.restype = res
+    lib.abi = abi
+    _libs[dtype] = lib
+    return lib
+
+
+class LE2DError(RuntimeError):
+    pass
+
+
+class Task:
+    """One le_task owned through the C ABI.  `grid()` returns a numpy view of
+    the host mirror, (ny, nx, 5) for AOS or (5, ny, nx) for SOA, after making
+    it current."""
+
+    def __init__(self, nx, ny, dt, h=(1.0, 1.0), mat=(2.0, 1.0, 1.0), stype=ST_AOS,
+                 dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.lib = load(self.dtype)
+        self.abi = self.lib.abi
+        self.mat = self.abi.le_material()
+        self.lib.le_init_material(mat[0], mat[1], mat[2], ctypes.byref(self.mat))
+        self.t = self.abi.le_task()
+        self.lib.le_init_task(ctypes.byref(self.t), dt, self.abi.le_vec2(*h), self.mat,
+                              self.abi.le_point2(nx, ny), stype)
+        self.nx, self.ny, self.stype = nx, ny, stype
+        self._open = True
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.t)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.le2d_last_error()
+            raise LE2DError(f"{what}: {rc} {msg.decode() if msg else ''}")
+
+    def view(self):
+        """Numpy view of the host mirror without synchronising."""
+        n = self.nx * self.ny * 5
+        flat = np.ctypeslib.as_array(self.t.grid, shape=(n,))
+        if self.stype == ST_AOS:
+            return flat.reshape(self.ny, self.nx, 5)
+        return flat.reshape(5, self.ny, self.nx)
+
+    def grid(self):
+        self.sync_host()
+        return self.view()
+
+    def load(self, state):
+        """Write `state` (same shape as view()) into the mirror and mark it dirty."""
+        self.sync_host()
+        self.view()[...] = state
+        self._check(self.lib.le_mark_host_dirty(self.ref), "mark_host_dirty")
+
+    def set_ball(self, c, r, s):
+        self.lib.le_set_ball(self.ref, self.abi.le_vec2(*c), r, s)
+
+    def step(self):
+        (self.lib.le_step if self.stype == ST_AOS else self.lib.le_step_soa)(self.ref)
+
+    def step_n(self, n):
+        self._check(self.lib.le_step_n(self.ref, n), "le_step_n")
+
+    def step_x(self):
+        (self.lib.le_step_x if self.stype == ST_AOS else self.lib.le_soa_step_x)(self.ref)
+
+    def step_y(self):
+        (self.lib.le_step_y if self.stype == ST_AOS else self.lib.le_soa_step_y)(self.ref)
+
+    def step_y_cf(self, cfs):
+        self.lib.le_step_y_cf(self.ref, cfs)
+
+    def step_cf(self, cfs):
+        self.lib.le_step_cf(self.ref, cfs)
+
+    def sync_host(self):
+        self._check(self.lib.le_sync_host(self.ref), "le_sync_host")
+
+    def wait(self):
+        self._check(self.lib.le2d_wait(self.ref), "le2d_wait")
+
+    def set_sync_mode(self, mode):
+        self._check(self.lib.le2d_set_sync_mode(self.ref, mode), "set_sync_mode")
+
+    def set_tiles(self, x_rows=0, y_seg=0, y_block=0):
+        self._check(self.lib.le2d_set_tiles(self.ref, x_rows, y_seg, y_block), "set_tiles")
+
+    def get_tiles(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.le2d_get_tiles(self.ref, ctypes.byref(a), ctypes.byref(b),
+                                            ctypes.byref(c)), "get_tiles")
+        return a.value, b.value, c.value
+
+    def time_sweeps(self, steps):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.le2d_time_sweeps(self.ref, steps, ctypes.byref(a),
+                                              ctypes.byref(b)), "time_sweeps")
+        return a.value, b.value
+
+    def save(self, path):
+        return self.lib.le_save_task(self.ref, path.encode() if isinstance(path, str) else path)
+
+    def device_bytes(self):
+        return self.lib.le2d_device_bytes(self.ref)
+
+    def close(self):
+        if self._open:
+            self.lib.le_free_task(self.ref)
+            self._open = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

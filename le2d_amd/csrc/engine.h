@@ -1,0 +1,76 @@
+// engine.h -- internal interface of the MI355X sweep engine.
+//
+// A grid lives in HBM as five pitched planes (vx, vy, sxx, sxy, syy), each
+// row starting on a 256-byte boundary, in a ping-pong pair: a sweep reads one
+// buffer and writes the other.  The C ABI (le_core_api.cpp) and the batched
+// and serving layers sit on top of this header.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace le2d {
+
+// y-edge handling.  CLAMP: positions past the last row read the last row
+// (reference simple/le_core.c:534).  RING: they read the stale slot of the
+// 5-row ring of the SOA y-sweep (simple/le_core.c:310-349) and le_step_y_cf
+// (:574-594), i.e. row max(p-5, 0) for position p >= ny.
+enum Edge { EDGE_CLAMP = 0, EDGE_RING = 1 };
+
+// Constants of one sweep in the arithmetic type T.  k = dt*c/h is the
+// Courant number of the +-c1 (k1) and +-c2 (k2) invariants; omk = 1 - k.
+template <typename T>
+struct SweepCoef {
+  T k1, k2, omk1, omk2;
+  T irhoc1, irhoc2, rhoc1, rhoc2, rhoc3;
+};
+
+// Launch shape of the sweeps (chosen per grid by the autotuner).
+struct Tiles {
+  int x_rows;   // rows per x-sweep workgroup
+  int y_seg;    // rows per y-sweep segment (each segment re-reads 4 halo rows)
+  int y_block;  // columns per y-sweep workgroup (64, 128 or 256)
+};
+
+struct DevGrid {
+  int elem = 0;        // bytes per value: 4 or 8
+  int nx = 0, ny = 0;
+  long pitch = 0;      // elements between rows
+  long plane = 0;      // elements between planes
+  void *buf[2] = {nullptr, nullptr};
+  int cur = 0;         // buffer that holds the current state
+  hipStream_t stream = nullptr;
+  int device = 0;
+  Tiles tiles{8, 128, 256};
+};
+
+// Error reporting: every function returns 0 on success or a nonzero code and
+// records a message retrievable with last_error().
+enum Status { OK = 0, ERR_ARG = 1, ERR_DEVICE = 2, ERR_NOMEM = 3, ERR_STATE = 4, ERR_BUDGET = 5 };
+int set_error(int code, const char *fmt, ...);
+const char *last_error();
+
+// Aborts with a message when no usable HIP device is present.
+int require_device();
+
+long plane_pitch(int nx, int elem);
+size_t grid_bytes(int elem, int nx, int ny);  // device bytes of one state buffer
+int grid_alloc(DevGrid *g, int elem, int nx, int ny, hipStream_t stream);
+void grid_free(DevGrid *g);
+
+// Host <-> device transfers of the current state.  AOS host arrays hold
+// nx*ny nodes of five values; SOA host arrays hold five planes of nx*ny.
+int grid_upload(DevGrid *g, const void *host, int soa);
+int grid_download(DevGrid *g, void *host, int soa);
+
+// One sweep of the current state into the other buffer, then swap.
+template <typename T>
+int sweep_x(DevGrid *g, const SweepCoef<T> &c);
+template <typename T>
+int sweep_y(DevGrid *g, const SweepCoef<T> &c, int edge);
+
+Tiles default_tiles(int elem, int nx, int ny);
+
+}  // namespace le2d

@@ -1,0 +1,204 @@
+// engine.hip -- device memory, host<->device transfers and error state of
+// the sweep engine.
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "engine.h"
+
+namespace le2d {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+const char *last_error() { return g_err; }
+
+int require_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1) {
+    fprintf(stderr,
+            "le2d: no HIP device is available (%s); this library runs the sweeps on the GPU "
+            "only and has no CPU path\n",
+            e != hipSuccess ? hipGetErrorString(e) : "device count 0");
+    abort();
+  }
+  return OK;
+}
+
+// Rows start on 256-byte boundaries so every wavefront row access is aligned.
+long plane_pitch(int nx, int elem) {
+  const long per = 256 / elem;
+  return ((long)nx + per - 1) / per * per;
+}
+
+// Planes are separated by the pitched plane plus a 4 KiB skew, so the five
+// streams of a sweep do not start on the same HBM channel at equal offsets.
+static long plane_stride(int nx, int ny, int elem) {
+  return plane_pitch(nx, elem) * ny + 4096 / elem;
+}
+
+size_t grid_bytes(int elem, int nx, int ny) {
+  return (size_t)plane_stride(nx, ny, elem) * 5 * elem;
+}
+
+int grid_alloc(DevGrid *g, int elem, int nx, int ny, hipStream_t stream) {
+  if ((elem != 4 && elem != 8) || nx < 3 || ny < 3)
+    return set_error(ERR_ARG, "grid_alloc: need elem 4|8 and nx, ny >= 3 (got %d, %d, %d)", elem,
+                     nx, ny);
+  g->elem = elem;
+  g->nx = nx;
+  g->ny = ny;
+  g->pitch = plane_pitch(nx, elem);
+  g->plane = plane_stride(nx, ny, elem);
+  g->stream = stream;
+  g->cur = 0;
+  (void)hipGetDevice(&g->device);
+  const size_t bytes = grid_bytes(elem, nx, ny);
+  for (int k = 0; k < 2; ++k) {
+    hipError_t e = hipMalloc(&g->buf[k], bytes);
+    if (e != hipSuccess) {
+      g->buf[k] = nullptr;
+      grid_free(g);
+      return set_error(ERR_NOMEM, "grid_alloc: hipMalloc(%zu) failed: %s", bytes,
+                       hipGetErrorString(e));
+    }
+  }
+  g->tiles = default_tiles(elem, nx, ny);
+  return OK;
+}
+
+void grid_free(DevGrid *g) {
+  for (int k = 0; k < 2; ++k) {
+    if (g->buf[k]) (void)hipFree(g->buf[k]);
+    g->buf[k] = nullptr;
+  }
+}
+
+// AOS (node-contiguous, unpitched) <-> pitched planes.  One lane per node;
+// the AOS side is read/written as 5 consecutive values per lane.
+template <typename T>
+__global__ void k_aos_to_planes(const T *__restrict__ aos, T *__restrict__ pl, int nx, int ny,
+                                long pitch, long plane) {
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (long)nx * ny) return;
+  const long j = n / nx, i = n - j * nx;
+  const T *s = aos + 5 * n;
+  T *d = pl + j * pitch + i;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) d[q * plane] = s[q];
+}
+
+template <typename T>
+__global__ void k_planes_to_aos(const T *__restrict__ pl, T *__restrict__ aos, int nx, int ny,
+                                long pitch, long plane) {
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (long)nx * ny) return;
+  const long j = n / nx, i = n - j * nx;
+  const T *s = pl + j * pitch + i;
+  T *d = aos + 5 * n;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) d[q] = s[q * plane];
+}
+
+static int check(hipError_t e, const char *what) {
+  if (e != hipSuccess) return set_error(ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+  return OK;
+}
+
+// The other ping-pong buffer stages the AOS image (it holds 5*plane >= 5*nx*ny
+// values and carries no state between steps).
+int grid_upload(DevGrid *g, const void *host, int soa) {
+  const size_t es = g->elem;
+  const size_t row = (size_t)g->nx * es;
+  void *cur = g->buf[g->cur];
+  if (soa) {
+    for (int q = 0; q < 5; ++q) {
+      const char *h = static_cast<const char *>(host) + (size_t)q * g->nx * g->ny * es;
+      char *d = static_cast<char *>(cur) + (size_t)q * g->plane * es;
+      int rc = check(hipMemcpy2DAsync(d, g->pitch * es, h, row, row, g->ny, hipMemcpyHostToDevice,
+                                      g->stream),
+                     "upload planes");
+      if (rc) return rc;
+    }
+    return check(hipStreamSynchronize(g->stream), "upload sync");
+  }
+  void *stage = g->buf[g->cur ^ 1];
+  const size_t bytes = (size_t)g->nx * g->ny * 5 * es;
+  int rc = check(hipMemcpyAsync(stage, host, bytes, hipMemcpyHostToDevice, g->stream), "upload");
+  if (rc) return rc;
+  const long n = (long)g->nx * g->ny;
+  const int bs = 256;
+  const unsigned nb = (unsigned)((n + bs - 1) / bs);
+  if (g->elem == 8)
+    hipLaunchKernelGGL(k_aos_to_planes<double>, dim3(nb), dim3(bs), 0, g->stream,
+                       static_cast<const double *>(stage), static_cast<double *>(cur), g->nx, g->ny,
+                       g->pitch, g->plane);
+  else
+    hipLaunchKernelGGL(k_aos_to_planes<float>, dim3(nb), dim3(bs), 0, g->stream,
+                       static_cast<const float *>(stage), static_cast<float *>(cur), g->nx, g->ny,
+                       g->pitch, g->plane);
+  rc = check(hipGetLastError(), "aos_to_planes launch");
+  if (rc) return rc;
+  return check(hipStreamSynchronize(g->stream), "upload sync");
+}
+
+int grid_download(DevGrid *g, void *host, int soa) {
+  const size_t es = g->elem;
+  const size_t row = (size_t)g->nx * es;
+  void *cur = g->buf[g->cur];
+  if (soa) {
+    for (int q = 0; q < 5; ++q) {
+      char *h = static_cast<char *>(host) + (size_t)q * g->nx * g->ny * es;
+      const char *d = static_cast<const char *>(cur) + (size_t)q * g->plane * es;
+      int rc = check(hipMemcpy2DAsync(h, row, d, g->pitch * es, row, g->ny, hipMemcpyDeviceToHost,
+                                      g->stream),
+                     "download planes");
+      if (rc) return rc;
+    }
+    return check(hipStreamSynchronize(g->stream), "download sync");
+  }
+  void *stage = g->buf[g->cur ^ 1];
+  const long n = (long)g->nx * g->ny;
+  const int bs = 256;
+  const unsigned nb = (unsigned)((n + bs - 1) / bs);
+  if (g->elem == 8)
+    hipLaunchKernelGGL(k_planes_to_aos<double>, dim3(nb), dim3(bs), 0, g->stream,
+                       static_cast<const double *>(cur), static_cast<double *>(stage), g->nx, g->ny,
+                       g->pitch, g->plane);
+  else
+    hipLaunchKernelGGL(k_planes_to_aos<float>, dim3(nb), dim3(bs), 0, g->stream,
+                       static_cast<const float *>(cur), static_cast<float *>(stage), g->nx, g->ny,
+                       g->pitch, g->plane);
+  int rc = check(hipGetLastError(), "planes_to_aos launch");
+  if (rc) return rc;
+  rc = check(hipMemcpyAsync(host, stage, (size_t)n * 5 * es, hipMemcpyDeviceToHost, g->stream),
+             "download");
+  if (rc) return rc;
+  return check(hipStreamSynchronize(g->stream), "download sync");
+}
+
+Tiles default_tiles(int elem, int nx, int ny) {
+  Tiles t;
+  t.x_rows = 4;
+  t.y_block = nx >= 8 * 256 ? 256 : (nx >= 8 * 128 ? 128 : 64);
+  // enough segments for ~8 workgroups per CU, but at least 32 rows each
+  const long cols = (nx + t.y_block - 1) / t.y_block;
+  long want = (2048 + cols - 1) / cols;
+  long seg = (ny + want - 1) / want;
+  if (seg < 32) seg = 32;
+  if (seg > ny) seg = ny;
+  t.y_seg = (int)seg;
+  (void)elem;
+  return t;
+}
+
+}  // namespace le2d

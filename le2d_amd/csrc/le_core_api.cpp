@@ -1,0 +1,448 @@
+// le_core_api.cpp -- the le_core.h C ABI over the HIP sweep engine.
+//
+// Compiled twice: with -DUSE_DOUBLE into liblecore_d.so and without into
+// liblecore_f.so, exactly as the reference builds `real` (simple/Makefile:6-7).
+// Host-side arithmetic (material constants, disk test, VTK magnitudes) is the
+// reference's, operation for operation; the sweeps run on the GPU.
+//
+// Device state is found through a table keyed by the host mirror pointer
+// (task->grid), because le_task has no spare field and callers may copy the
+// struct by value.
+#include <hip/hip_runtime.h>
+
+#include <cassert>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+
+#include "engine.h"
+#include "le2d.h"
+#include "le_core.h"
+
+namespace {
+
+using le2d::DevGrid;
+using le2d::SweepCoef;
+
+struct TaskState {
+  DevGrid g;
+  bool dev_valid = false;   // device buffers hold a state
+  bool host_dirty = true;   // mirror newer than device
+  bool dev_newer = false;   // device newer than mirror
+  bool pinned = false;      // mirror from hipHostMalloc
+  bool owned = true;        // mirror allocated by le_init_task
+  int mode = LE2D_SYNC_WAIT;
+  hipStream_t stream = nullptr;
+};
+
+std::mutex g_mu;
+std::unordered_map<const real *, TaskState *> g_tasks;
+
+int env_mode() {
+  const char *s = getenv("LE2D_SYNC");
+  if (!s) return LE2D_SYNC_WAIT;
+  if (!strcmp(s, "async")) return LE2D_SYNC_ASYNC;
+  if (!strcmp(s, "coherent")) return LE2D_SYNC_COHERENT;
+  return LE2D_SYNC_WAIT;
+}
+
+bool have_device() {
+  int n = 0;
+  return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+}
+
+[[noreturn]] void die(const char *what, int rc) {
+  fprintf(stderr, "le2d: %s failed (%d): %s\n", what, rc, le2d::last_error());
+  abort();
+}
+
+TaskState *find(const le_task *t) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_tasks.find(t->grid);
+  if (it != g_tasks.end()) return it->second;
+  // A task whose mirror was not allocated by le_init_task: adopt it.
+  TaskState *s = new TaskState();
+  s->owned = false;
+  s->mode = env_mode();
+  g_tasks[t->grid] = s;
+  return s;
+}
+
+// Make the device hold the current state (allocate on first use, upload
+// host edits).
+int ensure_device(const le_task *t, TaskState *s) {
+  le2d::require_device();
+  if (!s->dev_valid) {
+    if (!s->stream) {
+      hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+      if (e != hipSuccess)
+        return le2d::set_error(LE2D_ERR_DEVICE, "stream: %s", hipGetErrorString(e));
+    }
+    int rc = le2d::grid_alloc(&s->g, (int)sizeof(real), t->n.x, t->n.y, s->stream);
+    if (rc) return rc;
+    s->dev_valid = true;
+    s->host_dirty = true;
+  }
+  if (s->host_dirty) {
+    int rc = le2d::grid_upload(&s->g, t->grid, t->stype == ST_SOA);
+    if (rc) return rc;
+    s->host_dirty = false;
+    s->dev_newer = false;
+  }
+  return LE2D_OK;
+}
+
+int sync_host(le_task *t, TaskState *s) {
+  if (!s->dev_valid || !s->dev_newer) return LE2D_OK;
+  int rc = le2d::grid_download(&s->g, t->grid, t->stype == ST_SOA);
+  if (rc) return rc;
+  s->dev_newer = false;
+  return LE2D_OK;
+}
+
+// Sweep constants as the reference forms them (simple/le_core.c:468-469,
+// :511-512): k = dt*c/h in `real`; the (1 - k) of tvd2 is formed in double.
+SweepCoef<real> coef(const le_task *t, real h) {
+  SweepCoef<real> c;
+  c.k1 = t->dt * t->mat.c1 / h;
+  c.k2 = t->dt * t->mat.c2 / h;
+  c.omk1 = (real)(1.0 - c.k1);
+  c.omk2 = (real)(1.0 - c.k2);
+  c.irhoc1 = t->mat.irhoc1;
+  c.irhoc2 = t->mat.irhoc2;
+  c.rhoc1 = t->mat.rhoc1;
+  c.rhoc2 = t->mat.rhoc2;
+  c.rhoc3 = t->mat.rhoc3;
+  return c;
+}
+
+enum { DO_X = 1, DO_Y = 2 };
+
+// Queue sweeps on the task's stream, then honour the sync mode.
+int run(le_task *t, int what, int edge, int steps) {
+  TaskState *s = find(t);
+  int rc = ensure_device(t, s);
+  if (rc) return rc;
+  const SweepCoef<real> cx = coef(t, t->h.x), cy = coef(t, t->h.y);
+  for (int k = 0; k < steps; ++k) {
+    if (what & DO_X) {
+      rc = le2d::sweep_x<real>(&s->g, cx);
+      if (rc) return rc;
+    }
+    if (what & DO_Y) {
+      rc = le2d::sweep_y<real>(&s->g, cy, edge);
+      if (rc) return rc;
+    }
+  }
+  s->dev_newer = true;
+  if (s->mode == LE2D_SYNC_ASYNC) return LE2D_OK;
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return le2d::set_error(LE2D_ERR_DEVICE, "step: %s", hipGetErrorString(e));
+  if (s->mode == LE2D_SYNC_COHERENT) return sync_host(t, s);
+  return LE2D_OK;
+}
+
+void run_or_die(le_task *t, int what, int edge, const char *name) {
+  int rc = run(t, what, edge, 1);
+  if (rc) die(name, rc);
+}
+
+}  // namespace
+
+extern "C" {
+
+// simple/le_core.c:171-187, same expressions in `real` with double literals.
+void le_init_material(const real c1, const real c2, const real rho, le_material *m) {
+  m->c1 = c1;
+  m->c2 = c2;
+  m->rho = rho;
+  m->irhoc1 = 1.0 / (c1 * rho);
+  m->irhoc2 = 1.0 / (c2 * rho);
+  m->rhoc1 = c1 * rho;
+  m->rhoc2 = c2 * rho;
+  const real mu = rho * c2 * c2;
+  const real la = rho * c1 * c1 - 2.0 * mu;
+  m->rhoc3 = rho * c1 * la / (la + 2.0 * mu);
+}
+
+// simple/le_core.c:114-123.  The mirror is pinned when a device exists so
+// uploads and downloads run at DMA speed.
+void le_init_task(le_task *task, const real dt, const le_vec2 h, const le_material mat,
+                  const le_point2 n, const int stype) {
+  task->dt = dt;
+  task->h = h;
+  task->mat = mat;
+  task->n = n;
+  task->stype = stype;
+  const size_t bytes = sizeof(le_node) * (size_t)n.x * (size_t)n.y;
+  void *p = nullptr;
+  bool pinned = false;
+  if (bytes && have_device() && hipHostMalloc(&p, bytes, hipHostMallocDefault) == hipSuccess) {
+    pinned = true;
+  } else {
+    p = malloc(bytes ? bytes : 1);
+  }
+  if (!p) {
+    fprintf(stderr, "le2d: le_init_task: cannot allocate %zu bytes\n", bytes);
+    abort();
+  }
+  memset(p, 0, bytes);
+  task->grid = static_cast<real *>(p);
+  TaskState *s = new TaskState();
+  s->pinned = pinned;
+  s->mode = env_mode();
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_tasks.find(task->grid);
+  if (it != g_tasks.end()) delete it->second;
+  g_tasks[task->grid] = s;
+}
+
+// simple/le_core.c:125-128.
+void le_free_task(le_task *task) {
+  TaskState *s = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tasks.find(task->grid);
+    if (it != g_tasks.end()) {
+      s = it->second;
+      g_tasks.erase(it);
+    }
+  }
+  if (s) {
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    le2d::grid_free(&s->g);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+  }
+  if (!s || s->owned) {
+    if (s && s->pinned)
+      (void)hipHostFree(task->grid);
+    else
+      free(task->grid);
+  }
+  delete s;
+}
+
+// simple/le_core.c:71-94.  Loop order and arithmetic as the reference; the
+// distance is libm's double sqrt of a sum formed in `real`.
+void le_set_ball(le_task *t, const le_vec2 c, const real r, const real s) {
+  TaskState *st = find(t);
+  if (sync_host(t, st)) die("le_set_ball (download)", LE2D_ERR_DEVICE);
+  const long nx = t->n.x, ny = t->n.y;
+  for (int i = 0; i < t->n.x; i++) {
+    for (int j = 0; j < t->n.y; j++) {
+      const le_vec2 x = {t->h.x * i, t->h.y * j};
+      const le_vec2 d = {x.x - c.x, x.y - c.y};
+      if (::sqrt((double)(d.x * d.x + d.y * d.y)) < r) {
+        if (t->stype == ST_AOS) {
+          le_node *nd = reinterpret_cast<le_node *>(t->grid) + (i + j * nx);
+          nd->s.xx = s;
+          nd->s.yy = s;
+        } else if (t->stype == ST_SOA) {
+          t->grid[2 * nx * ny + i + j * nx] = s;
+          t->grid[4 * nx * ny + i + j * nx] = s;
+        } else {
+          assert(0);
+        }
+      }
+    }
+  }
+  st->host_dirty = true;
+}
+
+// simple/le_core.c:99-169: header lines, then |v| as big-endian float32,
+// rows j outer, nodes i inner.
+int le_save_task(le_task *t, const char *file) {
+  TaskState *st = find(t);
+  int rc = sync_host(t, st);
+  if (rc) die("le_save_task (download)", rc);
+  FILE *fp = fopen(file, "w");
+  if (fp == NULL) {
+    perror("Failed to open file");
+    return 1;
+  }
+  fprintf(fp, "# vtk DataFile Version 3.0\n");
+  fprintf(fp, "Created by le_save_task\n");
+  fprintf(fp, "BINARY\n");
+  fprintf(fp, "DATASET STRUCTURED_POINTS\n");
+  fprintf(fp, "DIMENSIONS %d %d 1\n", t->n.x, t->n.y);
+  fprintf(fp, "SPACING %f %f 0.0\n", (double)t->h.x, (double)t->h.y);
+  fprintf(fp, "ORIGIN 0.0 0.0 0.0\n");
+  fprintf(fp, "POINT_DATA %d\n", (int)((unsigned)t->n.x * (unsigned)t->n.y));
+  fprintf(fp, "SCALARS v float 1\n");
+  fprintf(fp, "LOOKUP_TABLE v_table\n");
+  const long nx = t->n.x, ny = t->n.y;
+  unsigned char *buf = static_cast<unsigned char *>(malloc((size_t)nx * 4));
+  for (long j = 0; j < ny; j++) {
+    for (long i = 0; i < nx; i++) {
+      real vx, vy;
+      if (t->stype == ST_AOS) {
+        const le_node *nd = reinterpret_cast<const le_node *>(t->grid) + (i + j * nx);
+        vx = nd->v.x;
+        vy = nd->v.y;
+      } else {
+        vx = t->grid[i + j * nx];
+        vy = t->grid[nx * ny + i + j * nx];
+      }
+      const float v = (float)::sqrt((double)(vx * vx + vy * vy));
+      uint32_t bits;
+      memcpy(&bits, &v, 4);
+      buf[4 * i + 0] = (unsigned char)(bits >> 24);
+      buf[4 * i + 1] = (unsigned char)(bits >> 16);
+      buf[4 * i + 2] = (unsigned char)(bits >> 8);
+      buf[4 * i + 3] = (unsigned char)bits;
+    }
+    fwrite(buf, 1, (size_t)nx * 4, fp);
+  }
+  free(buf);
+  fclose(fp);
+  return 0;
+}
+
+// simple/le_core.c:435 (asserts AOS, :437).
+void le_step_x(le_task *task) {
+  assert(task->stype == ST_AOS);
+  run_or_die(task, DO_X, le2d::EDGE_CLAMP, "le_step_x");
+}
+
+// simple/le_core.c:503 (asserts AOS, :505).
+void le_step_y(le_task *task) {
+  assert(task->stype == ST_AOS);
+  run_or_die(task, DO_Y, le2d::EDGE_CLAMP, "le_step_y");
+}
+
+// simple/le_core.c:605-620.
+void le_step(le_task *task) {
+  assert(task->stype == ST_AOS);
+  run_or_die(task, DO_X | DO_Y, le2d::EDGE_CLAMP, "le_step");
+}
+
+// simple/le_core.c:540-602: column blocks of width cfs; the result does not
+// depend on cfs, only on the ring edge of its 5-row window.
+void le_step_y_cf(le_task *task, const int_t cfs) {
+  assert(task->stype == ST_AOS);
+  assert(cfs > 0 && cfs <= task->n.x);
+  run_or_die(task, DO_Y, le2d::EDGE_RING, "le_step_y_cf");
+}
+
+// Declared at simple/le_core.h:248, never defined by the reference; the
+// natural composition of its pieces.
+void le_step_cf(le_task *task, const int cfs) {
+  assert(task->stype == ST_AOS);
+  assert(cfs > 0 && cfs <= task->n.x);
+  run_or_die(task, DO_X, le2d::EDGE_CLAMP, "le_step_cf");
+  run_or_die(task, DO_Y, le2d::EDGE_RING, "le_step_cf");
+}
+
+// simple/le_core.c:367-433.
+void le_soa_step_x(le_task *task) {
+  assert(task->stype == ST_SOA);
+  run_or_die(task, DO_X, le2d::EDGE_CLAMP, "le_soa_step_x");
+}
+
+// simple/le_core.c:259-364.
+void le_soa_step_y(le_task *task) {
+  assert(task->stype == ST_SOA);
+  run_or_die(task, DO_Y, le2d::EDGE_RING, "le_soa_step_y");
+}
+
+// simple/le_core.c:623-627.
+void le_step_soa(le_task *task) {
+  assert(task->stype == ST_SOA);
+  run_or_die(task, DO_X | DO_Y, le2d::EDGE_RING, "le_step_soa");
+}
+
+// ---------------------------------------------------------------------------
+// le2d.h extensions.
+
+const char *le2d_last_error(void) { return le2d::last_error(); }
+
+int le2d_set_sync_mode(le_task *task, int mode) {
+  if (mode < LE2D_SYNC_WAIT || mode > LE2D_SYNC_COHERENT)
+    return le2d::set_error(LE2D_ERR_ARG, "sync mode %d", mode);
+  find(task)->mode = mode;
+  return LE2D_OK;
+}
+
+int le2d_wait(le_task *task) {
+  TaskState *s = find(task);
+  if (!s->stream) return LE2D_OK;
+  hipError_t e = hipStreamSynchronize(s->stream);
+  return e == hipSuccess ? LE2D_OK
+                         : le2d::set_error(LE2D_ERR_DEVICE, "wait: %s", hipGetErrorString(e));
+}
+
+int le_sync_host(le_task *task) { return sync_host(task, find(task)); }
+
+int le_mark_host_dirty(le_task *task) {
+  find(task)->host_dirty = true;
+  return LE2D_OK;
+}
+
+int le_step_n(le_task *task, int steps) {
+  if (steps < 0) return le2d::set_error(LE2D_ERR_ARG, "steps %d", steps);
+  const int edge = task->stype == ST_SOA ? le2d::EDGE_RING : le2d::EDGE_CLAMP;
+  return run(task, DO_X | DO_Y, edge, steps);
+}
+
+int le2d_set_tiles(le_task *task, int x_rows, int y_seg, int y_block) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  if (x_rows < 0 || y_seg < 0 || (y_block && y_block != 64 && y_block != 128 && y_block != 256))
+    return le2d::set_error(LE2D_ERR_ARG, "tiles %d %d %d", x_rows, y_seg, y_block);
+  if (x_rows) s->g.tiles.x_rows = x_rows;
+  if (y_seg) s->g.tiles.y_seg = y_seg;
+  if (y_block) s->g.tiles.y_block = y_block;
+  return LE2D_OK;
+}
+
+int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  *x_rows = s->g.tiles.x_rows;
+  *y_seg = s->g.tiles.y_seg;
+  *y_block = s->g.tiles.y_block;
+  return LE2D_OK;
+}
+
+int le2d_time_sweeps(le_task *task, int steps, double *ms_x, double *ms_y) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  if (steps < 1) return le2d::set_error(LE2D_ERR_ARG, "steps %d", steps);
+  const int edge = task->stype == ST_SOA ? le2d::EDGE_RING : le2d::EDGE_CLAMP;
+  const SweepCoef<real> cx = coef(task, task->h.x), cy = coef(task, task->h.y);
+  hipEvent_t ev[3];
+  for (auto &e : ev) (void)hipEventCreate(&e);
+  double tx = 0, ty = 0;
+  for (int k = 0; k < steps && !rc; ++k) {
+    (void)hipEventRecord(ev[0], s->stream);
+    rc = le2d::sweep_x<real>(&s->g, cx);
+    (void)hipEventRecord(ev[1], s->stream);
+    if (!rc) rc = le2d::sweep_y<real>(&s->g, cy, edge);
+    (void)hipEventRecord(ev[2], s->stream);
+    (void)hipEventSynchronize(ev[2]);
+    float a = 0, b = 0;
+    (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+    tx += a;
+    ty += b;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  s->dev_newer = true;
+  if (rc) return rc;
+  *ms_x = tx / steps;
+  *ms_y = ty / steps;
+  return LE2D_OK;
+}
+
+size_t le2d_device_bytes(le_task *task) {
+  TaskState *s = find(task);
+  return s->dev_valid ? 2 * le2d::grid_bytes(s->g.elem, s->g.nx, s->g.ny) : 0;
+}
+
+}  // extern "C"

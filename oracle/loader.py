@@ -1,0 +1,148 @@
+"""ctypes loaders for the oracle and for the compiled reference (test infrastructure only)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EDGE_CLAMP = 0
+EDGE_RING = 1
+
+
+def build_oracle(ref=False):
+    """Compile the oracle (and, if asked and the sources exist, the reference)."""
+    subprocess.check_call(["make", "-s", "-B", "-C", HERE, "all"])
+    if ref and os.path.isdir("/root/reference/simple"):
+        subprocess.check_call(["make", "-s", "-B", "-C", HERE, "ref"])
+
+
+def _ptr(a, real):
+    return a.ctypes.data_as(ctypes.POINTER(real))
+
+
+class Oracle:
+    """Numpy front end of le2d_ref.c.  Grids are numpy arrays shaped (ny, nx, 5)
+    (AOS, node (i, j) at [j, i]) or (5, ny, nx) (SOA planes)."""
+
+    def __init__(self, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.real = ctypes.c_double if self.dtype == np.float64 else ctypes.c_float
+        name = "liboracle_d.so" if self.dtype == np.float64 else "liboracle_f.so"
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            build_oracle()
+        self.lib = ctypes.CDLL(path)
+        R, P = self.real, ctypes.POINTER
+        self.Mat = type("orc_material", (ctypes.Structure,),
+                        {"_fields_": [(n, R) for n in ("c1", "c2", "rho", "irhoc1", "irhoc2",
+                                                        "rhoc1", "rhoc2", "rhoc3")]})
+        L = self.lib
+        L.orc_material_init.argtypes = [R, R, R, P(self.Mat)]
+        L.orc_superbee.argtypes = [R]
+        L.orc_superbee.restype = R
+        L.orc_tvd2.argtypes = [R] * 5
+        L.orc_tvd2.restype = R
+        grid_args = [P(R), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_sweep_x.argtypes = grid_args + [R, R, P(self.Mat)]
+        L.orc_sweep_y.argtypes = grid_args + [R, R, P(self.Mat), ctypes.c_int]
+        L.orc_step.argtypes = grid_args + [R, R, R, P(self.Mat)]
+        L.orc_run.argtypes = grid_args + [R, R, R, P(self.Mat), ctypes.c_int]
+        L.orc_set_ball.argtypes = grid_args + [R] * 6
+        L.orc_set_ball.restype = ctypes.c_long
+        L.orc_save_vtk.argtypes = [P(R), ctypes.c_int, ctypes.c_int, ctypes.c_int, R, R,
+                                   ctypes.c_char_p]
+        assert L.orc_real_size() == self.dtype.itemsize
+
+    # --- scalar pieces -----------------------------------------------------
+    def material(self, c1, c2, rho):
+        m = self.Mat()
+        self.lib.orc_material_init(c1, c2, rho, ctypes.byref(m))
+        return m
+
+    def superbee(self, r):
+        return self.lib.orc_superbee(r)
+
+    def tvd2(self, c, a, b, u, d):
+        return self.lib.orc_tvd2(c, a, b, u, d)
+
+    # --- grid operations (in place) ----------------------------------------
+    def _grid(self, g):
+        assert g.dtype == self.dtype and g.flags.c_contiguous
+        if g.shape[-1] == 5 and g.ndim == 3:
+            return g.shape[1], g.shape[0], 0
+        assert g.ndim == 3 and g.shape[0] == 5
+        return g.shape[2], g.shape[1], 1
+
+    def _mat(self, mat):
+        return mat if isinstance(mat, self.Mat) else self.material(*mat)
+
+    def sweep_x(self, g, dt, hx, mat):
+        nx, ny, soa = self._grid(g)
+        rc = self.lib.orc_sweep_x(_ptr(g, self.real), nx, ny, soa, dt, hx,
+                                  ctypes.byref(self._mat(mat)))
+        assert rc == 0
+
+    def sweep_y(self, g, dt, hy, mat, edge=EDGE_CLAMP):
+        nx, ny, soa = self._grid(g)
+        rc = self.lib.orc_sweep_y(_ptr(g, self.real), nx, ny, soa, dt, hy,
+                                  ctypes.byref(self._mat(mat)), edge)
+        assert rc == 0
+
+    def run(self, g, dt, h, mat, steps):
+        nx, ny, soa = self._grid(g)
+        rc = self.lib.orc_run(_ptr(g, self.real), nx, ny, soa, dt, h[0], h[1],
+                              ctypes.byref(self._mat(mat)), steps)
+        assert rc == 0
+
+    def set_ball(self, g, h, c, r, s):
+        nx, ny, soa = self._grid(g)
+        return self.lib.orc_set_ball(_ptr(g, self.real), nx, ny, soa, h[0], h[1], c[0], c[1],
+                                     r, s)
+
+    def save_vtk(self, g, h, path):
+        nx, ny, soa = self._grid(g)
+        return self.lib.orc_save_vtk(_ptr(g, self.real), nx, ny, soa, h[0], h[1],
+                                     path.encode() if isinstance(path, str) else path)
+
+
+class Reference:
+    """The reference's simple/ path (oracle/_ref/libsimple_{d,f}.so), driven
+    through its own le_core.h entry points with a ctypes le_task."""
+
+    def __init__(self, dtype=np.float64):
+        import sys
+        sys.path.insert(0, os.path.dirname(HERE))
+        from le2d_amd.abi import ABI_D, ABI_F
+        self.dtype = np.dtype(dtype)
+        self.abi = ABI_D if self.dtype == np.float64 else ABI_F
+        name = "libsimple_d.so" if self.dtype == np.float64 else "libsimple_f.so"
+        path = os.path.join(HERE, "_ref", name)
+        if not os.path.exists(path):
+            raise FileNotFoundError(path + " (run `make -C oracle ref` where the reference "
+                                    "sources are present)")
+        self.lib = self.abi.bind(ctypes.CDLL(path))
+
+    @staticmethod
+    def available(dtype=np.float64):
+        name = "libsimple_d.so" if np.dtype(dtype) == np.float64 else "libsimple_f.so"
+        return os.path.exists(os.path.join(HERE, "_ref", name))
+
+    def material(self, c1, c2, rho):
+        m = self.abi.le_material()
+        self.lib.le_init_material(c1, c2, rho, ctypes.byref(m))
+        return m
+
+    def task(self, nx, ny, dt, h, mat, stype=1):
+        t = self.abi.le_task()
+        if not isinstance(mat, self.abi.le_material):
+            mat = self.material(*mat)
+        self.lib.le_init_task(ctypes.byref(t), dt, self.abi.le_vec2(*h), mat,
+                              self.abi.le_point2(nx, ny), stype)
+        return t
+
+    def view(self, t):
+        """Numpy view of the task grid: (ny, nx, 5) for AOS, (5, ny, nx) for SOA."""
+        nx, ny = t.n.x, t.n.y
+        flat = np.ctypeslib.as_array(t.grid, shape=(nx * ny * 5,))
+        return flat.reshape(ny, nx, 5) if t.stype == 1 else flat.reshape(5, ny, nx)
