@@ -54,10 +54,12 @@ def cpu_baseline(n, budget_s=12.0):
         o.set_ball(g, (1.0, 1.0), (cx, cy), 64.0, 1.0)
         step = lambda: o.run(g, 0.25, (1.0, 1.0), (2.0, 1.0, 1.0), 1)  # noqa: E731
         kind, done = "port", lambda: None  # noqa: E731
+    step()  # first touch of the host pages
     t0 = time.perf_counter()
     step()
-    one = time.perf_counter() - t0
-    steps = max(1, min(50, int(budget_s / max(one, 1e-3))))
+    step()
+    one = (time.perf_counter() - t0) / 2
+    steps = max(1, min(200, int(budget_s / max(one, 1e-3))))
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
@@ -65,8 +67,18 @@ def cpu_baseline(n, budget_s=12.0):
     done()
     return {"value": n * n * steps / el / 1e9, "unit": "Gnode-updates/s", "cores": cores,
             "kind": kind,
-            "sample": f"{n}x{n} FP64, {steps} steps after 1 warm-up step, "
+            "sample": f"{n}x{n} FP64, {steps} steps after 3 warm-up steps, "
                       f"{el:.1f} s wall, OMP_NUM_THREADS={cores}"}
+
+
+def measured_traffic(n, dtype):
+    """HBM bytes per k_step_xy launch from the committed rocprofv3 counter
+    summary for this workload (profiles/traffic_k_step_xy_<n>_<dtype>.json),
+    or None when no such measurement exists."""
+    p = os.path.join(ROOT, "profiles", f"traffic_k_step_xy_{n}_{dtype}.json")
+    if not os.path.exists(p):
+        return None
+    return json.load(open(p)).get("hbm_bytes_per_launch")
 
 
 def main():
@@ -102,19 +114,28 @@ def main():
     nodes = n * n
     value = nodes * args.steps / el / 1e9
 
-    # per-sweep durations with HIP events on the task's stream
+    # Per-launch durations with HIP events on the task's stream.  le_step runs
+    # one kernel per step (k_step_xy); the separate sweep kernels (le_step_x,
+    # le_step_y) are timed too for reference.
+    ms_step = t.time_steps(args.profile_steps)
     ms_x, ms_y = t.time_sweeps(args.profile_steps)
-    bytes_per_sweep = nodes * 10 * es  # 5 reads + 5 writes per node
-    dom = ("sweep_y", ms_y) if ms_y >= ms_x else ("sweep_x", ms_x)
-    achieved = bytes_per_sweep / (dom[1] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1),
+    # Algorithmic bytes per node-update: 2 sweeps x (5 reads + 5 writes) x
+    # sizeof(real); one k_step_xy launch advances every node by one step.
+    alg_bytes = nodes * 20 * es
+    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
+    traffic = measured_traffic(n, args.dtype)
+    roofline = {"bound": "hbm", "kernel": "k_step_xy", "achieved": round(achieved, 1),
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                "traffic": None,
-                "ms_per_launch": {"sweep_x": round(ms_x, 4), "sweep_y": round(ms_y, 4)},
-                "algorithmic_bytes_per_launch": bytes_per_sweep}
-    step_gbs = nodes * 20 * es / (el / args.steps) / 1e9
+                "traffic": traffic,
+                "algorithmic_bytes_per_launch": alg_bytes,
+                "ms_per_launch": {"k_step_xy": round(ms_step, 4), "k_sweep_x": round(ms_x, 4),
+                                  "k_sweep_y": round(ms_y, 4)},
+                "two_kernel_GBps": {"k_sweep_x": round(nodes * 10 * es / ms_x / 1e6, 1),
+                                    "k_sweep_y": round(nodes * 10 * es / ms_y / 1e6, 1)}}
     tiles = t.get_tiles()
     t.close()
+    copy_ms = le2d_amd.copy_probe(nodes * 5 * es, 10)
+    copy_gbs = 2 * nodes * 5 * es / copy_ms / 1e6
 
     cpu = None
     if not args.no_cpu:
@@ -130,8 +151,8 @@ def main():
         "config": {"workload": f"plain split-step sweep, {n}x{n} grid, disk r=64 at "
                                f"({centre[0]:.0f},{centre[1]:.0f}), c1=2 c2=1 rho=1, dt=0.25",
                    "nx": n, "ny": n, "tiles": {"x_rows": tiles[0], "y_seg": tiles[1],
-                                               "y_block": tiles[2]}},
-        "hbm_equiv_GBps_per_step": round(step_gbs, 1),
+                                               "y_block": tiles[2], "xy_seg": tiles[3]}},
+        "hbm_copy_GBps": round(copy_gbs, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -55,15 +55,22 @@ int le_mark_host_dirty(le_task *task);
 /* `steps` full steps (x then y), one completion wait at the end. */
 int le_step_n(le_task *task, int steps);
 
-/* Launch shape of the sweeps: rows per x-sweep workgroup, rows per y-sweep
- * segment, columns per y-sweep workgroup (64, 128, 256).  Zero keeps the
- * current value.  Results do not depend on the shape. */
-int le2d_set_tiles(le_task *task, int x_rows, int y_seg, int y_block);
-int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block);
+/* Launch shape: rows per x-sweep workgroup, rows per y-sweep segment,
+ * columns per y-sweep workgroup (64, 128, 256), rows per segment of the fused
+ * step.  Zero keeps the current value.  Results do not depend on the shape. */
+int le2d_set_tiles(le_task *task, int x_rows, int y_seg, int y_block, int xy_seg);
+int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block, int *xy_seg);
 
-/* Run `steps` steps, timing every sweep launch with HIP events on the task's
- * stream; returns the mean duration of one x-sweep and one y-sweep in ms. */
+/* Run `steps` steps as separate x- and y-sweep launches, timing each launch
+ * with HIP events on the task's stream; mean ms of one x- and one y-sweep. */
 int le2d_time_sweeps(le_task *task, int steps, double *ms_x, double *ms_y);
+/* Run `steps` fused steps (the path le_step takes), timing each launch; mean
+ * ms of one step. */
+int le2d_time_steps(le_task *task, int steps, double *ms_step);
+
+/* Streaming device-to-device copy of `bytes`, `iters` times; mean ms per
+ * copy.  Measures the achievable HBM bandwidth on this device. */
+int le2d_copy_probe(size_t bytes, int iters, double *ms);
 
 /* Device bytes held by the task (both state buffers). */
 size_t le2d_device_bytes(le_task *task);

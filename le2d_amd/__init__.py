@@ -27,11 +27,14 @@ _EXT = {
     "le_sync_host": (["T"], ctypes.c_int),
     "le_mark_host_dirty": (["T"], ctypes.c_int),
     "le_step_n": (["T", ctypes.c_int], ctypes.c_int),
-    "le2d_set_tiles": (["T", ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
-    "le2d_get_tiles": (["T", ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
-                        ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "le2d_set_tiles": (["T", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+                       ctypes.c_int),
+    "le2d_get_tiles": (["T"] + [ctypes.POINTER(ctypes.c_int)] * 4, ctypes.c_int),
     "le2d_time_sweeps": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "le2d_time_steps": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "le2d_copy_probe": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
+                        ctypes.c_int),
     "le2d_device_bytes": (["T"], ctypes.c_size_t),
 }
 
@@ -67,6 +70,16 @@ def load(dtype=np.float64):
 
 class LE2DError(RuntimeError):
     pass
+
+
+def copy_probe(nbytes, iters=20, dtype=np.float64):
+    """Mean ms of one device-to-device copy of `nbytes` (achievable HBM rate)."""
+    lib = load(dtype)
+    ms = ctypes.c_double()
+    rc = lib.le2d_copy_probe(nbytes, iters, ctypes.byref(ms))
+    if rc != 0:
+        raise LE2DError(f"copy_probe: {rc} {lib.le2d_last_error().decode()}")
+    return ms.value
 
 
 class Task:
@@ -144,20 +157,26 @@ class Task:
     def set_sync_mode(self, mode):
         self._check(self.lib.le2d_set_sync_mode(self.ref, mode), "set_sync_mode")
 
-    def set_tiles(self, x_rows=0, y_seg=0, y_block=0):
-        self._check(self.lib.le2d_set_tiles(self.ref, x_rows, y_seg, y_block), "set_tiles")
+    def set_tiles(self, x_rows=0, y_seg=0, y_block=0, xy_seg=0):
+        self._check(self.lib.le2d_set_tiles(self.ref, x_rows, y_seg, y_block, xy_seg),
+                    "set_tiles")
 
     def get_tiles(self):
-        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        self._check(self.lib.le2d_get_tiles(self.ref, ctypes.byref(a), ctypes.byref(b),
-                                            ctypes.byref(c)), "get_tiles")
-        return a.value, b.value, c.value
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(self.lib.le2d_get_tiles(self.ref, *[ctypes.byref(x) for x in v]),
+                    "get_tiles")
+        return tuple(x.value for x in v)
 
     def time_sweeps(self, steps):
         a, b = ctypes.c_double(), ctypes.c_double()
         self._check(self.lib.le2d_time_sweeps(self.ref, steps, ctypes.byref(a),
                                               ctypes.byref(b)), "time_sweeps")
         return a.value, b.value
+
+    def time_steps(self, steps):
+        a = ctypes.c_double()
+        self._check(self.lib.le2d_time_steps(self.ref, steps, ctypes.byref(a)), "time_steps")
+        return a.value
 
     def save(self, path):
         return self.lib.le_save_task(self.ref, path.encode() if isinstance(path, str) else path)

@@ -32,6 +32,7 @@ struct Tiles {
   int x_rows;   // rows per x-sweep workgroup
   int y_seg;    // rows per y-sweep segment (each segment re-reads 4 halo rows)
   int y_block;  // columns per y-sweep workgroup (64, 128 or 256)
+  int xy_seg;   // rows per segment of the fused x+y step
 };
 
 struct DevGrid {
@@ -43,7 +44,7 @@ struct DevGrid {
   int cur = 0;         // buffer that holds the current state
   hipStream_t stream = nullptr;
   int device = 0;
-  Tiles tiles{8, 128, 256};
+  Tiles tiles{4, 128, 256, 128};
 };
 
 // Error reporting: every function returns 0 on success or a nonzero code and
@@ -70,6 +71,14 @@ template <typename T>
 int sweep_x(DevGrid *g, const SweepCoef<T> &c);
 template <typename T>
 int sweep_y(DevGrid *g, const SweepCoef<T> &c, int edge);
+// One full step (x-sweep then y-sweep) in a single pass.
+template <typename T>
+int step_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy, int edge);
+
+// Streaming copy of `bytes` between two device buffers, `iters` times; mean
+// milliseconds per copy (HIP events).  The reference point for achievable
+// HBM bandwidth and for scaling hardware byte counters.
+int copy_probe(size_t bytes, int iters, double *ms);
 
 Tiles default_tiles(int elem, int nx, int ny);
 

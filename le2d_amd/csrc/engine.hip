@@ -189,6 +189,15 @@ int grid_download(DevGrid *g, void *host, int soa) {
 Tiles default_tiles(int elem, int nx, int ny) {
   Tiles t;
   t.x_rows = 4;
+  {
+    // fused step: ~8192 single-wave workgroups, segments of at least 32 rows
+    const long strips = (nx + 63) / 64;
+    const long want = (8192 + strips - 1) / strips;
+    long seg = (ny + want - 1) / want;
+    if (seg < 32) seg = 32;
+    if (seg > ny) seg = ny;
+    t.xy_seg = (int)seg;
+  }
   t.y_block = nx >= 8 * 256 ? 256 : (nx >= 8 * 128 ? 128 : 64);
   // enough segments for ~8 workgroups per CU, but at least 32 rows each
   const long cols = (nx + t.y_block - 1) / t.y_block;
@@ -199,6 +208,45 @@ Tiles default_tiles(int elem, int nx, int ny) {
   t.y_seg = (int)seg;
   (void)elem;
   return t;
+}
+
+__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long n) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+int copy_probe(size_t bytes, int iters, double *ms) {
+  require_device();
+  if (bytes < 16 || iters < 1) return set_error(ERR_ARG, "copy_probe: bytes %zu iters %d", bytes, iters);
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    if (a) (void)hipFree(a);
+    return set_error(ERR_NOMEM, "copy_probe: hipMalloc(%zu)", bytes);
+  }
+  (void)hipMemset(a, 1, bytes);
+  const long n = (long)(bytes / 16);
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const unsigned blocks = (unsigned)cus * 16;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, 0, (const uint4 *)a, (uint4 *)b, n);
+  (void)hipEventRecord(e0, 0);
+  for (int k = 0; k < iters; ++k)
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, 0, (const uint4 *)a, (uint4 *)b, n);
+  (void)hipEventRecord(e1, 0);
+  hipError_t e = hipEventSynchronize(e1);
+  float t = 0;
+  (void)hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return set_error(ERR_DEVICE, "copy_probe: %s", hipGetErrorString(e));
+  *ms = t / iters;
+  return OK;
 }
 
 }  // namespace le2d

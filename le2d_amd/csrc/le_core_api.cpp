@@ -128,6 +128,11 @@ int run(le_task *t, int what, int edge, int steps) {
   if (rc) return rc;
   const SweepCoef<real> cx = coef(t, t->h.x), cy = coef(t, t->h.y);
   for (int k = 0; k < steps; ++k) {
+    if (what == (DO_X | DO_Y)) {
+      rc = le2d::step_xy<real>(&s->g, cx, cy, edge);  // one pass per step
+      if (rc) return rc;
+      continue;
+    }
     if (what & DO_X) {
       rc = le2d::sweep_x<real>(&s->g, cx);
       if (rc) return rc;
@@ -387,26 +392,59 @@ int le_step_n(le_task *task, int steps) {
   return run(task, DO_X | DO_Y, edge, steps);
 }
 
-int le2d_set_tiles(le_task *task, int x_rows, int y_seg, int y_block) {
+int le2d_set_tiles(le_task *task, int x_rows, int y_seg, int y_block, int xy_seg) {
   TaskState *s = find(task);
   int rc = ensure_device(task, s);
   if (rc) return rc;
-  if (x_rows < 0 || y_seg < 0 || (y_block && y_block != 64 && y_block != 128 && y_block != 256))
-    return le2d::set_error(LE2D_ERR_ARG, "tiles %d %d %d", x_rows, y_seg, y_block);
+  if (x_rows < 0 || y_seg < 0 || xy_seg < 0 ||
+      (y_block && y_block != 64 && y_block != 128 && y_block != 256))
+    return le2d::set_error(LE2D_ERR_ARG, "tiles %d %d %d %d", x_rows, y_seg, y_block, xy_seg);
   if (x_rows) s->g.tiles.x_rows = x_rows;
   if (y_seg) s->g.tiles.y_seg = y_seg;
   if (y_block) s->g.tiles.y_block = y_block;
+  if (xy_seg) s->g.tiles.xy_seg = xy_seg;
   return LE2D_OK;
 }
 
-int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block) {
+int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block, int *xy_seg) {
   TaskState *s = find(task);
   int rc = ensure_device(task, s);
   if (rc) return rc;
   *x_rows = s->g.tiles.x_rows;
   *y_seg = s->g.tiles.y_seg;
   *y_block = s->g.tiles.y_block;
+  *xy_seg = s->g.tiles.xy_seg;
   return LE2D_OK;
+}
+
+int le2d_time_steps(le_task *task, int steps, double *ms_step) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  if (steps < 1) return le2d::set_error(LE2D_ERR_ARG, "steps %d", steps);
+  const int edge = task->stype == ST_SOA ? le2d::EDGE_RING : le2d::EDGE_CLAMP;
+  const SweepCoef<real> cx = coef(task, task->h.x), cy = coef(task, task->h.y);
+  hipEvent_t ev[2];
+  for (auto &e : ev) (void)hipEventCreate(&e);
+  double tot = 0;
+  for (int k = 0; k < steps && !rc; ++k) {
+    (void)hipEventRecord(ev[0], s->stream);
+    rc = le2d::step_xy<real>(&s->g, cx, cy, edge);
+    (void)hipEventRecord(ev[1], s->stream);
+    (void)hipEventSynchronize(ev[1]);
+    float a = 0;
+    (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+    tot += a;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  s->dev_newer = true;
+  if (rc) return rc;
+  *ms_step = tot / steps;
+  return LE2D_OK;
+}
+
+int le2d_copy_probe(size_t bytes, int iters, double *ms) {
+  return le2d::copy_probe(bytes, iters, ms);
 }
 
 int le2d_time_sweeps(le_task *task, int steps, double *ms_x, double *ms_y) {

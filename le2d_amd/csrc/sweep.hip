@@ -101,60 +101,10 @@ __device__ __forceinline__ void apply(T &vn, T &vt, T &sn, T &so, T &sxy, const 
 }
 
 // ---------------------------------------------------------------------------
-// x-sweep.  A workgroup of BX lanes owns BX consecutive nodes of a row and
-// walks `rows` rows.  Per row: coalesced loads of the five planes, invariants
-// to LDS (with a 2-node clamped halo each side), one face per lane (lane 0
-// also the segment's left face) to LDS, then the node update and coalesced
-// stores.  LDS is double-buffered by row parity, so a row costs two barriers.
-template <typename T, int BX>
-__global__ __launch_bounds__(BX) void k_sweep_x(const T *__restrict__ src, T *__restrict__ dst,
-                                                int nx, int ny, long pitch, long plane,
-                                                int rows, SweepCoef<T> c) {
-  __shared__ W4<T> sw[2][BX + 4];
-  __shared__ W4<T> sf[2][BX + 1];
-  const int t = threadIdx.x;
-  const int x0 = blockIdx.x * BX;
-  const int x = x0 + t;
-  const int xc = min(x, nx - 1);
-  // lanes 0..3 also fetch the halo nodes x0-2, x0-1, x0+BX, x0+BX+1 (clamped)
-  const int hx = (t < 2) ? max(x0 - 2 + t, 0) : min(x0 + BX + (t - 2), nx - 1);
-  const int hs = (t < 2) ? t : BX + t;
-  const int j0 = blockIdx.y * rows;
-  const int j1 = min(j0 + rows, ny);
-  int b = 0;
-  for (int j = j0; j < j1; ++j, b ^= 1) {
-    const T *r = src + (long)j * pitch;
-    T vx = r[xc], vy = r[plane + xc], sxx = r[2 * plane + xc], sxy = r[3 * plane + xc],
-      syy = r[4 * plane + xc];
-    const W4<T> w = invariants(vx, sxx, vy, sxy, c);
-    sw[b][t + 2] = w;
-    if (t < 4) {
-      sw[b][hs] = invariants(r[hx], r[2 * plane + hx], r[plane + hx], r[3 * plane + hx], c);
-    }
-    __syncthreads();
-    // face x+1/2 -> sf[t+1]; lane 0 also face x0-1/2 -> sf[0]
-    sf[b][t + 1] = faces(sw[b][t + 1], sw[b][t + 2], sw[b][t + 3], sw[b][t + 4], c);
-    if (t == 0) sf[b][0] = faces(sw[b][0], sw[b][1], sw[b][2], sw[b][3], c);
-    __syncthreads();
-    if (x < nx) {
-      apply(vx, vy, sxx, syy, sxy, w, sf[b][t], sf[b][t + 1], c);
-      T *o = dst + (long)j * pitch + x;
-      o[0] = vx;
-      o[plane] = vy;
-      o[2 * plane] = sxx;
-      o[3 * plane] = sxy;
-      o[4 * plane] = syy;
-    }
-  }
-}
+// Rows and nodes.
 
-// ---------------------------------------------------------------------------
-// y-sweep.  Each lane owns one column and marches down a segment of `seg`
-// rows with a register window of invariants (positions j-1 .. j+2) and the
-// raw nodes of rows j .. j+2; every step reads one new row and writes one
-// row, so a wavefront's accesses are contiguous row pieces (no strided
-// loads, no transpose).  The face flux at j+1/2 computed in step j is reused
-// as the inflow face in step j+1.  Segments re-read two rows above and below.
+// Row of window position p along y: clamp (le_step) or the ring edge
+// (le_step_soa / le_step_y_cf).
 template <int EDGE>
 __device__ __forceinline__ int ypos(int p, int ny) {
   if (p < 0) return 0;
@@ -178,11 +128,100 @@ __device__ __forceinline__ Node<T> load_node(const T *__restrict__ p, long plane
   return n;
 }
 
+// Loads and stores through a wavefront-uniform row base (kept in SGPRs) and a
+// 32-bit per-lane byte offset, which maps to the SGPR-base + VGPR-offset
+// addressing of global memory instructions and keeps VGPRs free of 64-bit
+// addresses.  Planes are < 4 GiB, so the offset always fits.
+template <typename T>
+__device__ __forceinline__ T ld_u(const T *__restrict__ base, unsigned boff) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+
+template <typename T>
+__device__ __forceinline__ void st_u(T *__restrict__ base, unsigned boff, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+
+template <typename T>
+__device__ __forceinline__ Node<T> load_row_node(const T *__restrict__ row, long plane,
+                                                 unsigned boff) {
+  Node<T> n;
+  n.vx = ld_u(row, boff);
+  n.vy = ld_u(row + plane, boff);
+  n.sxx = ld_u(row + 2 * plane, boff);
+  n.sxy = ld_u(row + 3 * plane, boff);
+  n.syy = ld_u(row + 4 * plane, boff);
+  return n;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_row_node(T *__restrict__ row, long plane, unsigned boff,
+                                               const Node<T> &u) {
+  st_u(row, boff, u.vx);
+  st_u(row + plane, boff, u.vy);
+  st_u(row + 2 * plane, boff, u.sxx);
+  st_u(row + 3 * plane, boff, u.sxy);
+  st_u(row + 4 * plane, boff, u.syy);
+}
+
 template <typename T>
 __device__ __forceinline__ W4<T> inv_y(const Node<T> &n, const SweepCoef<T> &c) {
   return invariants(n.vy, n.syy, n.vx, n.sxy, c);
 }
 
+// ---------------------------------------------------------------------------
+// x-sweep.  A workgroup of BX lanes owns BX consecutive nodes of a row and
+// walks `rows` rows.  Per row: coalesced loads of the five planes, invariants
+// to LDS (with a 2-node clamped halo each side), one face per lane (lane 0
+// also the segment's left face) to LDS, then the node update and coalesced
+// stores.  LDS is double-buffered by row parity, so a row costs two barriers.
+template <typename T, int BX>
+__global__ __launch_bounds__(BX) void k_sweep_x(const T *__restrict__ src, T *__restrict__ dst,
+                                                int nx, int ny, long pitch, long plane,
+                                                int rows, SweepCoef<T> c) {
+  __shared__ W4<T> sw[2][BX + 4];
+  __shared__ W4<T> sf[2][BX + 1];
+  const int t = threadIdx.x;
+  const int x0 = blockIdx.x * BX;
+  const int x = x0 + t;
+  const int xc = min(x, nx - 1);
+  // lanes 0..3 also fetch the halo nodes x0-2, x0-1, x0+BX, x0+BX+1 (clamped)
+  const int hx = (t < 2) ? max(x0 - 2 + t, 0) : min(x0 + BX + (t - 2), nx - 1);
+  const int hs = (t < 2) ? t : BX + t;
+  const int j0 = blockIdx.y * rows;
+  const int j1 = min(j0 + rows, ny);
+  const unsigned boff = (unsigned)xc * (unsigned)sizeof(T);
+  const unsigned hoff = (unsigned)hx * (unsigned)sizeof(T);
+  const unsigned woff = (unsigned)x * (unsigned)sizeof(T);
+  int b = 0;
+  for (int j = j0; j < j1; ++j, b ^= 1) {
+    const T *r = src + (long)j * pitch;
+    Node<T> u = load_row_node(r, plane, boff);
+    const W4<T> w = invariants(u.vx, u.sxx, u.vy, u.sxy, c);
+    sw[b][t + 2] = w;
+    if (t < 4) {
+      const Node<T> h = load_row_node(r, plane, hoff);
+      sw[b][hs] = invariants(h.vx, h.sxx, h.vy, h.sxy, c);
+    }
+    __syncthreads();
+    // face x+1/2 -> sf[t+1]; lane 0 also face x0-1/2 -> sf[0]
+    sf[b][t + 1] = faces(sw[b][t + 1], sw[b][t + 2], sw[b][t + 3], sw[b][t + 4], c);
+    if (t == 0) sf[b][0] = faces(sw[b][0], sw[b][1], sw[b][2], sw[b][3], c);
+    __syncthreads();
+    if (x < nx) {
+      apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, sf[b][t], sf[b][t + 1], c);
+      store_row_node(dst + (long)j * pitch, plane, woff, u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y-sweep.  Each lane owns one column and marches down a segment of `seg`
+// rows with a register window of invariants (positions j-1 .. j+2) and the
+// raw nodes of rows j .. j+2; every step reads one new row and writes one
+// row, so a wavefront's accesses are contiguous row pieces (no strided
+// loads, no transpose).  The face flux at j+1/2 computed in step j is reused
+// as the inflow face in step j+1.  Segments re-read two rows above and below.
 template <typename T, int EDGE>
 __global__ __launch_bounds__(256) void k_sweep_y(const T *__restrict__ src, T *__restrict__ dst,
                                                  int nx, int ny, long pitch, long plane, int seg,
@@ -191,30 +230,25 @@ __global__ __launch_bounds__(256) void k_sweep_y(const T *__restrict__ src, T *_
   if (x >= nx) return;  // no barriers below
   const int j0 = blockIdx.y * seg;
   const int j1 = min(j0 + seg, ny);
-  const T *col = src + x;
-  T *out = dst + x;
+  const unsigned boff = (unsigned)x * (unsigned)sizeof(T);
+#define YROW(p) load_row_node(src + (long)ypos<EDGE>((p), ny) * pitch, plane, boff)
 
   // prologue: positions j0-2 .. j0+2
-  const W4<T> wm2 = inv_y(load_node(col + (long)ypos<EDGE>(j0 - 2, ny) * pitch, plane), c);
-  W4<T> wm1 = inv_y(load_node(col + (long)ypos<EDGE>(j0 - 1, ny) * pitch, plane), c);
-  Node<T> u0 = load_node(col + (long)ypos<EDGE>(j0, ny) * pitch, plane);
-  Node<T> u1 = load_node(col + (long)ypos<EDGE>(j0 + 1, ny) * pitch, plane);
-  Node<T> u2 = load_node(col + (long)ypos<EDGE>(j0 + 2, ny) * pitch, plane);
+  const W4<T> wm2 = inv_y(YROW(j0 - 2), c);
+  W4<T> wm1 = inv_y(YROW(j0 - 1), c);
+  Node<T> u0 = YROW(j0);
+  Node<T> u1 = YROW(j0 + 1);
+  Node<T> u2 = YROW(j0 + 2);
   W4<T> w0 = inv_y(u0, c), w1 = inv_y(u1, c), w2 = inv_y(u2, c);
   W4<T> fin = faces(wm2, wm1, w0, w1, c);  // face j0-1/2
 
   for (int j = j0; j < j1; ++j) {
     // issue the read of position j+3 before this row's arithmetic
-    const Node<T> un = load_node(col + (long)ypos<EDGE>(j + 3, ny) * pitch, plane);
+    const Node<T> un = YROW(j + 3);
     const W4<T> fout = faces(wm1, w0, w1, w2, c);  // face j+1/2
     Node<T> u = u0;
     apply(u.vy, u.vx, u.syy, u.sxx, u.sxy, w0, fin, fout, c);
-    T *o = out + (long)j * pitch;
-    o[0] = u.vx;
-    o[plane] = u.vy;
-    o[2 * plane] = u.sxx;
-    o[3 * plane] = u.sxy;
-    o[4 * plane] = u.syy;
+    store_row_node(dst + (long)j * pitch, plane, boff, u);
     fin = fout;
     wm1 = w0;
     w0 = w1;
@@ -224,10 +258,136 @@ __global__ __launch_bounds__(256) void k_sweep_y(const T *__restrict__ src, T *_
     u2 = un;
     w2 = inv_y(un, c);
   }
+#undef YROW
+}
+
+// ---------------------------------------------------------------------------
+// Fused step: the x-sweep and the y-sweep of one time step in one pass over
+// HBM (5 reads + 5 writes per node-update instead of twice that).
+//
+// A workgroup of FB lanes owns a window of FB consecutive columns starting two
+// columns left of its strip of FB-4 output columns, and a segment of `seg`
+// rows.  Lane t reads column x0-2+t (clamped into the grid), so the two halo
+// columns on each side of the strip come from ordinary lanes: no extra loads,
+// no divergent work.  Walking positions p = j0-2 .. j1+1 down the strip, each
+// position's row (ypos-mapped) is read once and x-swept through LDS (every
+// face of the strip in one pass over lanes 2..FB-2); each lane's post-x node
+// then feeds the register y-march of k_sweep_y.  Only rows [j0, j1) of the
+// FB-4 inner columns are written.  The y-march consumes x-swept values of the
+// rows just outside the segment, recomputed here from the unswept input,
+// which is exactly what the y-sweep of the two-kernel sequence reads.
+//
+// Workgroups are renumbered so that the tiles one XCD receives (the hardware
+// deals workgroups to the 8 XCDs round-robin) are consecutive strips of the
+// same segments: strips that share halo columns share an L2.
+__device__ __forceinline__ int xcd_tile(int L, int N) {
+  const int per = N >> 3, rem = N & 7, xcd = L & 7, k = L >> 3;
+  return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_node(T *p, long plane, const Node<T> &u) {
+  p[0] = u.vx;
+  p[plane] = u.vy;
+  p[2 * plane] = u.sxx;
+  p[3 * plane] = u.sxy;
+  p[4 * plane] = u.syy;
+}
+
+// x-sweep of one window row; returns the lane's post-x node (meaningful on
+// lanes 2..FB-3).  sw[t] holds lane t's invariants, sf[t] the face between
+// lanes t-1 and t.
+template <typename T, int FB>
+__device__ __forceinline__ Node<T> window_xsweep(Node<T> u, W4<T> *sw, W4<T> *sf, int t,
+                                                 const SweepCoef<T> &c) {
+  const W4<T> w = invariants(u.vx, u.sxx, u.vy, u.sxy, c);
+  sw[t] = w;
+  __syncthreads();
+  if (t >= 2 && t <= FB - 2) sf[t] = faces(sw[t - 2], sw[t - 1], sw[t], sw[t + 1], c);
+  __syncthreads();
+  if (t >= 2 && t <= FB - 3) apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, sf[t], sf[t + 1], c);
+  return u;
+}
+
+template <typename T, int EDGE, int FB>
+__global__ __launch_bounds__(FB, 4) void k_step_xy(const T *__restrict__ src, T *__restrict__ dst,
+                                                int nx, int ny, long pitch, long plane, int seg,
+                                                int nstrips, SweepCoef<T> cx, SweepCoef<T> cy) {
+  __shared__ W4<T> sw[FB];
+  __shared__ W4<T> sf[FB];
+  const int t = threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int strip = tile % nstrips, segi = tile / nstrips;
+  const int x = strip * (FB - 4) - 2 + t;
+  const unsigned boff = (unsigned)min(max(x, 0), nx - 1) * (unsigned)sizeof(T);
+  const unsigned woff = (unsigned)max(x, 0) * (unsigned)sizeof(T);
+  const bool writer = t >= 2 && t <= FB - 3 && x < nx;
+  const int j0 = segi * seg;
+  const int j1 = min(j0 + seg, ny);
+#define ROW(p) load_row_node(src + (long)ypos<EDGE>((p), ny) * pitch, plane, boff)
+
+  // prologue: positions j0-2 .. j0+2; the row of position j0+3 is put in flight
+  const W4<T> wm2 = inv_y(window_xsweep<T, FB>(ROW(j0 - 2), sw, sf, t, cx), cy);
+  W4<T> wm1 = inv_y(window_xsweep<T, FB>(ROW(j0 - 1), sw, sf, t, cx), cy);
+  Node<T> u0 = window_xsweep<T, FB>(ROW(j0), sw, sf, t, cx);
+  Node<T> u1 = window_xsweep<T, FB>(ROW(j0 + 1), sw, sf, t, cx);
+  Node<T> u2 = window_xsweep<T, FB>(ROW(j0 + 2), sw, sf, t, cx);
+  Node<T> next = ROW(j0 + 3);
+  W4<T> w0 = inv_y(u0, cy), w1 = inv_y(u1, cy), w2 = inv_y(u2, cy);
+  W4<T> fin = faces(wm2, wm1, w0, w1, cy);  // face j0-1/2
+
+  for (int j = j0; j < j1; ++j) {
+    const Node<T> cur = next;
+    if (j + 2 < j1) next = ROW(j + 4);
+    const W4<T> fout = faces(wm1, w0, w1, w2, cy);  // face j+1/2
+    Node<T> u = u0;
+    apply(u.vy, u.vx, u.syy, u.sxx, u.sxy, w0, fin, fout, cy);
+    if (writer) store_row_node(dst + (long)j * pitch, plane, woff, u);
+    if (j + 1 < j1) {  // uniform over the workgroup
+      const Node<T> un = window_xsweep<T, FB>(cur, sw, sf, t, cx);
+      fin = fout;
+      wm1 = w0;
+      w0 = w1;
+      w1 = w2;
+      u0 = u1;
+      u1 = u2;
+      u2 = un;
+      w2 = inv_y(un, cy);
+    }
+  }
+#undef ROW
 }
 
 // ---------------------------------------------------------------------------
 // Launchers.
+
+template <typename T>
+int step_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy, int edge) {
+  if (g->elem != (int)sizeof(T)) return set_error(ERR_ARG, "step_xy: element size mismatch");
+  constexpr int FB = 64;
+  const int nstrips = (g->nx + FB - 5) / (FB - 4);
+  const int seg = max(1, g->tiles.xy_seg);
+  const long nsegs = (g->ny + seg - 1) / seg;
+  const long nblocks = nsegs * nstrips;
+  if (nblocks > 0x7fffffffL) return set_error(ERR_ARG, "step_xy: grid too large");
+  const T *src = static_cast<const T *>(g->buf[g->cur]);
+  T *dst = static_cast<T *>(g->buf[g->cur ^ 1]);
+  if (edge == EDGE_RING)
+    hipLaunchKernelGGL((k_step_xy<T, EDGE_RING, FB>), dim3((unsigned)nblocks), dim3(FB), 0,
+                       g->stream, src, dst, g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx,
+                       cy);
+  else
+    hipLaunchKernelGGL((k_step_xy<T, EDGE_CLAMP, FB>), dim3((unsigned)nblocks), dim3(FB), 0,
+                       g->stream, src, dst, g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx,
+                       cy);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(ERR_DEVICE, "step_xy launch: %s", hipGetErrorString(e));
+  g->cur ^= 1;
+  return OK;
+}
+
+template int step_xy<double>(DevGrid *, const SweepCoef<double> &, const SweepCoef<double> &, int);
+template int step_xy<float>(DevGrid *, const SweepCoef<float> &, const SweepCoef<float> &, int);
 
 template <typename T>
 int sweep_x(DevGrid *g, const SweepCoef<T> &c) {

@@ -52,6 +52,27 @@ def test_tiny_golden_all_entry_points():
             assert np.array_equal(np.moveaxis(t.grid(), 0, -1), data[k + "_soa10"]), k + " soa"
 
 
+def test_fused_step_equals_separate_sweeps():
+    """le_step (one fused kernel) and le_step_x + le_step_y (two kernels) agree
+    bit for bit, on both edge modes and with segments of 1 row."""
+    rng = np.random.default_rng(15)
+    g0 = rng.standard_normal((131, 250, 5))
+    for stype in (ST_AOS, ST_SOA):
+        init = g0 if stype == ST_AOS else np.ascontiguousarray(np.moveaxis(g0, -1, 0))
+        res = []
+        for fused in (True, False):
+            with make(init, 0.3, (1.0, 0.8), (2.0, 1.0, 1.5), stype) as t:
+                t.set_tiles(xy_seg=1 if fused else 0)
+                for _ in range(3):
+                    if fused:
+                        t.step()
+                    else:
+                        t.step_x()
+                        t.step_y()
+                res.append(t.grid().copy())
+        assert np.array_equal(res[0], res[1])
+
+
 def test_step_n_matches_repeated_steps():
     cases, data = gd.tiny(np.float64)
     c = cases[-3]
@@ -120,11 +141,15 @@ def test_tile_shapes_give_identical_bits():
     nx, ny = 1100, 700
     g0 = rng.standard_normal((ny, nx, 5))
     out = []
-    for x_rows, y_seg, y_block in [(1, 7, 64), (4, 64, 128), (16, 1000, 256), (3, 33, 256)]:
+    for x_rows, y_seg, y_block, xy_seg in [(1, 7, 64, 1), (4, 64, 128, 5), (16, 1000, 256, 64),
+                                           (3, 33, 256, 1000)]:
         with make(g0, 0.2, (1.0, 1.0), (2.0, 1.0, 1.5)) as t:
-            t.set_tiles(x_rows, y_seg, y_block)
-            assert t.get_tiles() == (x_rows, y_seg, y_block)
-            t.step_n(4)
+            t.set_tiles(x_rows, y_seg, y_block, xy_seg)
+            assert t.get_tiles() == (x_rows, y_seg, y_block, xy_seg)
+            t.step_n(2)  # fused step kernel
+            for _ in range(2):  # separate sweep kernels
+                t.step_x()
+                t.step_y()
             out.append(t.grid().copy())
     for o in out[1:]:
         assert np.array_equal(o, out[0])
