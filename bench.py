@@ -151,7 +151,8 @@ def main():
         "config": {"workload": f"plain split-step sweep, {n}x{n} grid, disk r=64 at "
                                f"({centre[0]:.0f},{centre[1]:.0f}), c1=2 c2=1 rho=1, dt=0.25",
                    "nx": n, "ny": n, "tiles": {"x_rows": tiles[0], "y_seg": tiles[1],
-                                               "y_block": tiles[2], "xy_seg": tiles[3]}},
+                                               "y_block": tiles[2], "xy_seg": tiles[3],
+                                               "xy_block": tiles[4]}},
         "hbm_copy_GBps": round(copy_gbs, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -57,9 +57,12 @@ int le_step_n(le_task *task, int steps);
 
 /* Launch shape: rows per x-sweep workgroup, rows per y-sweep segment,
  * columns per y-sweep workgroup (64, 128, 256), rows per segment of the fused
- * step.  Zero keeps the current value.  Results do not depend on the shape. */
-int le2d_set_tiles(le_task *task, int x_rows, int y_seg, int y_block, int xy_seg);
-int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block, int *xy_seg);
+ * step, lanes per fused-step workgroup (64, 128, 256; it writes that many
+ * columns less four).  Zero keeps the current value.  Results do not depend
+ * on the shape. */
+int le2d_set_tiles(le_task *task, int x_rows, int y_seg, int y_block, int xy_seg, int xy_block);
+int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block, int *xy_seg,
+                   int *xy_block);
 
 /* Run `steps` steps as separate x- and y-sweep launches, timing each launch
  * with HIP events on the task's stream; mean ms of one x- and one y-sweep. */
@@ -74,6 +77,14 @@ int le2d_copy_probe(size_t bytes, int iters, double *ms);
 
 /* Device bytes held by the task (both state buffers). */
 size_t le2d_device_bytes(le_task *task);
+
+/* Raw restart file of the full state: a 128-byte header (magic "LE2DRAW1",
+ * sizeof(real), nx, ny, stype, then dt, h and the eight material constants as
+ * doubles) and the five fields as planes of nx*ny little-endian reals.
+ * le2d_restore needs a task of the same size and real type; it also restores
+ * dt, h and the material.  A dump/restore round trip is bit-exact. */
+int le2d_dump(le_task *task, const char *path);
+int le2d_restore(le_task *task, const char *path);
 
 #ifdef __cplusplus
 }

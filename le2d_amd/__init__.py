@@ -27,15 +27,16 @@ _EXT = {
     "le_sync_host": (["T"], ctypes.c_int),
     "le_mark_host_dirty": (["T"], ctypes.c_int),
     "le_step_n": (["T", ctypes.c_int], ctypes.c_int),
-    "le2d_set_tiles": (["T", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
-                       ctypes.c_int),
-    "le2d_get_tiles": (["T"] + [ctypes.POINTER(ctypes.c_int)] * 4, ctypes.c_int),
+    "le2d_set_tiles": (["T"] + [ctypes.c_int] * 5, ctypes.c_int),
+    "le2d_get_tiles": (["T"] + [ctypes.POINTER(ctypes.c_int)] * 5, ctypes.c_int),
     "le2d_time_sweeps": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "le2d_time_steps": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "le2d_copy_probe": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                         ctypes.c_int),
     "le2d_device_bytes": (["T"], ctypes.c_size_t),
+    "le2d_dump": (["T", ctypes.c_char_p], ctypes.c_int),
+    "le2d_restore": (["T", ctypes.c_char_p], ctypes.c_int),
 }
 
 _libs = {}
@@ -157,12 +158,12 @@ class Task:
     def set_sync_mode(self, mode):
         self._check(self.lib.le2d_set_sync_mode(self.ref, mode), "set_sync_mode")
 
-    def set_tiles(self, x_rows=0, y_seg=0, y_block=0, xy_seg=0):
-        self._check(self.lib.le2d_set_tiles(self.ref, x_rows, y_seg, y_block, xy_seg),
+    def set_tiles(self, x_rows=0, y_seg=0, y_block=0, xy_seg=0, xy_block=0):
+        self._check(self.lib.le2d_set_tiles(self.ref, x_rows, y_seg, y_block, xy_seg, xy_block),
                     "set_tiles")
 
     def get_tiles(self):
-        v = [ctypes.c_int() for _ in range(4)]
+        v = [ctypes.c_int() for _ in range(5)]
         self._check(self.lib.le2d_get_tiles(self.ref, *[ctypes.byref(x) for x in v]),
                     "get_tiles")
         return tuple(x.value for x in v)
@@ -180,6 +181,12 @@ class Task:
 
     def save(self, path):
         return self.lib.le_save_task(self.ref, path.encode() if isinstance(path, str) else path)
+
+    def dump(self, path):
+        self._check(self.lib.le2d_dump(self.ref, path.encode()), "le2d_dump")
+
+    def restore(self, path):
+        self._check(self.lib.le2d_restore(self.ref, path.encode()), "le2d_restore")
 
     def device_bytes(self):
         return self.lib.le2d_device_bytes(self.ref)

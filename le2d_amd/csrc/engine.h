@@ -20,10 +20,10 @@ namespace le2d {
 enum Edge { EDGE_CLAMP = 0, EDGE_RING = 1 };
 
 // Constants of one sweep in the arithmetic type T.  k = dt*c/h is the
-// Courant number of the +-c1 (k1) and +-c2 (k2) invariants; omk = 1 - k.
+// Courant number of the +-c1 (k1) and +-c2 (k2) invariants; hk = (1 - k)/2.
 template <typename T>
 struct SweepCoef {
-  T k1, k2, omk1, omk2;
+  T k1, k2, hk1, hk2;
   T irhoc1, irhoc2, rhoc1, rhoc2, rhoc3;
 };
 
@@ -33,6 +33,7 @@ struct Tiles {
   int y_seg;    // rows per y-sweep segment (each segment re-reads 4 halo rows)
   int y_block;  // columns per y-sweep workgroup (64, 128 or 256)
   int xy_seg;   // rows per segment of the fused x+y step
+  int xy_block; // lanes per fused-step workgroup (64, 128, 256); writes xy_block-4 columns
 };
 
 struct DevGrid {
@@ -44,7 +45,7 @@ struct DevGrid {
   int cur = 0;         // buffer that holds the current state
   hipStream_t stream = nullptr;
   int device = 0;
-  Tiles tiles{4, 128, 256, 128};
+  Tiles tiles{4, 128, 256, 64, 128};
 };
 
 // Error reporting: every function returns 0 on success or a nonzero code and

@@ -190,11 +190,14 @@ Tiles default_tiles(int elem, int nx, int ny) {
   Tiles t;
   t.x_rows = 4;
   {
-    // fused step: ~8192 single-wave workgroups, segments of at least 32 rows
-    const long strips = (nx + 63) / 64;
-    const long want = (8192 + strips - 1) / strips;
+    // fused step: 256-lane windows (252 output columns) and ~64 workgroups
+    // per CU in segments of at least 16 rows (measured best on 8192^2: 16-24
+    // rows; short segments keep the halo rows of neighbouring segments in L2)
+    t.xy_block = 256;
+    const long strips = (nx + 251) / 252;
+    const long want = (16384 + strips - 1) / strips;
     long seg = (ny + want - 1) / want;
-    if (seg < 32) seg = 32;
+    if (seg < 16) seg = 16;
     if (seg > ny) seg = ny;
     t.xy_seg = (int)seg;
   }
@@ -210,9 +213,22 @@ Tiles default_tiles(int elem, int nx, int ny) {
   return t;
 }
 
-__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long n) {
+// Each lane moves four 16-byte vectors per pass, all loads issued before the
+// stores; `n` is a multiple of 4 * stride for the sizes the probe uses, the
+// tail is handled one vector at a time.
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+__global__ void k_copy16(const v4u *__restrict__ src, v4u *__restrict__ dst, long n) {
   const long stride = (long)gridDim.x * blockDim.x;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const v4u a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+    __builtin_nontemporal_store(a, &dst[i]);
+    __builtin_nontemporal_store(b, &dst[i + stride]);
+    __builtin_nontemporal_store(c, &dst[i + 2 * stride]);
+    __builtin_nontemporal_store(d, &dst[i + 3 * stride]);
+  }
+  for (; i < n; i += stride) dst[i] = src[i];
 }
 
 int copy_probe(size_t bytes, int iters, double *ms) {
@@ -232,10 +248,10 @@ int copy_probe(size_t bytes, int iters, double *ms) {
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, 0, (const uint4 *)a, (uint4 *)b, n);
+  hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, 0, (const v4u *)a, (v4u *)b, n);
   (void)hipEventRecord(e0, 0);
   for (int k = 0; k < iters; ++k)
-    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, 0, (const uint4 *)a, (uint4 *)b, n);
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, 0, (const v4u *)a, (v4u *)b, n);
   (void)hipEventRecord(e1, 0);
   hipError_t e = hipEventSynchronize(e1);
   float t = 0;

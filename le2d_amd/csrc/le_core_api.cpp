@@ -104,13 +104,14 @@ int sync_host(le_task *t, TaskState *s) {
 }
 
 // Sweep constants as the reference forms them (simple/le_core.c:468-469,
-// :511-512): k = dt*c/h in `real`; the (1 - k) of tvd2 is formed in double.
+// :511-512): k = dt*c/h in `real`; the (1 - k) of tvd2 (:65-66) is formed in
+// double and halved exactly.
 SweepCoef<real> coef(const le_task *t, real h) {
   SweepCoef<real> c;
   c.k1 = t->dt * t->mat.c1 / h;
   c.k2 = t->dt * t->mat.c2 / h;
-  c.omk1 = (real)(1.0 - c.k1);
-  c.omk2 = (real)(1.0 - c.k2);
+  c.hk1 = (real)(1.0 - c.k1) * (real)0.5;
+  c.hk2 = (real)(1.0 - c.k2) * (real)0.5;
   c.irhoc1 = t->mat.irhoc1;
   c.irhoc2 = t->mat.irhoc2;
   c.rhoc1 = t->mat.rhoc1;
@@ -392,21 +393,24 @@ int le_step_n(le_task *task, int steps) {
   return run(task, DO_X | DO_Y, edge, steps);
 }
 
-int le2d_set_tiles(le_task *task, int x_rows, int y_seg, int y_block, int xy_seg) {
+int le2d_set_tiles(le_task *task, int x_rows, int y_seg, int y_block, int xy_seg, int xy_block) {
   TaskState *s = find(task);
   int rc = ensure_device(task, s);
   if (rc) return rc;
-  if (x_rows < 0 || y_seg < 0 || xy_seg < 0 ||
-      (y_block && y_block != 64 && y_block != 128 && y_block != 256))
-    return le2d::set_error(LE2D_ERR_ARG, "tiles %d %d %d %d", x_rows, y_seg, y_block, xy_seg);
+  auto width_ok = [](int w) { return w == 0 || w == 64 || w == 128 || w == 256; };
+  if (x_rows < 0 || y_seg < 0 || xy_seg < 0 || !width_ok(y_block) || !width_ok(xy_block))
+    return le2d::set_error(LE2D_ERR_ARG, "tiles %d %d %d %d %d", x_rows, y_seg, y_block, xy_seg,
+                           xy_block);
   if (x_rows) s->g.tiles.x_rows = x_rows;
   if (y_seg) s->g.tiles.y_seg = y_seg;
   if (y_block) s->g.tiles.y_block = y_block;
   if (xy_seg) s->g.tiles.xy_seg = xy_seg;
+  if (xy_block) s->g.tiles.xy_block = xy_block;
   return LE2D_OK;
 }
 
-int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block, int *xy_seg) {
+int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block, int *xy_seg,
+                   int *xy_block) {
   TaskState *s = find(task);
   int rc = ensure_device(task, s);
   if (rc) return rc;
@@ -414,6 +418,7 @@ int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block, int *xy
   *y_seg = s->g.tiles.y_seg;
   *y_block = s->g.tiles.y_block;
   *xy_seg = s->g.tiles.xy_seg;
+  *xy_block = s->g.tiles.xy_block;
   return LE2D_OK;
 }
 
@@ -481,6 +486,96 @@ int le2d_time_sweeps(le_task *task, int steps, double *ms_x, double *ms_y) {
 size_t le2d_device_bytes(le_task *task) {
   TaskState *s = find(task);
   return s->dev_valid ? 2 * le2d::grid_bytes(s->g.elem, s->g.nx, s->g.ny) : 0;
+}
+
+// Raw state file: a 128-byte header (magic, sizeof(real), nx, ny, stype, then
+// dt, h.x, h.y and the eight material constants as doubles) followed by the
+// five fields as planes vx|vy|sxx|sxy|syy of nx*ny little-endian reals, node
+// (i, j) at i + j*nx.  Restoring it reproduces the dumped state bit for bit.
+namespace {
+const char kMagic[8] = {'L', 'E', '2', 'D', 'R', 'A', 'W', '1'};
+struct RawHeader {
+  char magic[8];
+  int32_t real_size, nx, ny, stype;
+  double p[11];  // dt, h.x, h.y, c1, c2, rho, irhoc1, irhoc2, rhoc1, rhoc2, rhoc3
+  char pad[128 - 24 - 88];
+};
+static_assert(sizeof(RawHeader) == 128, "raw header is 128 bytes");
+}  // namespace
+
+int le2d_dump(le_task *task, const char *path) {
+  TaskState *s = find(task);
+  int rc = sync_host(task, s);
+  if (rc) return rc;
+  FILE *fp = fopen(path, "wb");
+  if (!fp) return le2d::set_error(LE2D_ERR_ARG, "dump: cannot open %s", path);
+  RawHeader h;
+  memset(&h, 0, sizeof h);
+  memcpy(h.magic, kMagic, 8);
+  h.real_size = (int32_t)sizeof(real);
+  h.nx = task->n.x;
+  h.ny = task->n.y;
+  h.stype = task->stype;
+  const le_material &m = task->mat;
+  const double p[11] = {task->dt, task->h.x, task->h.y, m.c1, m.c2, m.rho,
+                        m.irhoc1, m.irhoc2, m.rhoc1, m.rhoc2, m.rhoc3};
+  memcpy(h.p, p, sizeof p);
+  bool ok = fwrite(&h, sizeof h, 1, fp) == 1;
+  const size_t n = (size_t)task->n.x * task->n.y;
+  if (task->stype == ST_SOA) {
+    ok = ok && fwrite(task->grid, sizeof(real), 5 * n, fp) == 5 * n;
+  } else {
+    real *buf = static_cast<real *>(malloc(n * sizeof(real)));
+    for (int q = 0; q < 5 && ok && buf; ++q) {
+      for (size_t k = 0; k < n; ++k) buf[k] = task->grid[5 * k + q];
+      ok = fwrite(buf, sizeof(real), n, fp) == n;
+    }
+    ok = ok && buf;
+    free(buf);
+  }
+  ok = (fclose(fp) == 0) && ok;
+  return ok ? LE2D_OK : le2d::set_error(LE2D_ERR_ARG, "dump: short write to %s", path);
+}
+
+int le2d_restore(le_task *task, const char *path) {
+  TaskState *s = find(task);
+  FILE *fp = fopen(path, "rb");
+  if (!fp) return le2d::set_error(LE2D_ERR_ARG, "restore: cannot open %s", path);
+  RawHeader h;
+  if (fread(&h, sizeof h, 1, fp) != 1 || memcmp(h.magic, kMagic, 8) != 0) {
+    fclose(fp);
+    return le2d::set_error(LE2D_ERR_ARG, "restore: %s is not a raw state file", path);
+  }
+  if (h.real_size != (int32_t)sizeof(real) || h.nx != task->n.x || h.ny != task->n.y) {
+    fclose(fp);
+    return le2d::set_error(LE2D_ERR_ARG,
+                           "restore: file holds %dx%d of %d-byte reals, task is %dx%d of %d",
+                           h.nx, h.ny, h.real_size, task->n.x, task->n.y, (int)sizeof(real));
+  }
+  const size_t n = (size_t)task->n.x * task->n.y;
+  real *buf = static_cast<real *>(malloc(5 * n * sizeof(real)));
+  bool ok = buf && fread(buf, sizeof(real), 5 * n, fp) == 5 * n;
+  fclose(fp);
+  if (!ok) {
+    free(buf);
+    return le2d::set_error(LE2D_ERR_ARG, "restore: %s is truncated", path);
+  }
+  task->dt = (real)h.p[0];
+  task->h.x = (real)h.p[1];
+  task->h.y = (real)h.p[2];
+  real *mp[8] = {&task->mat.c1, &task->mat.c2, &task->mat.rho, &task->mat.irhoc1,
+                 &task->mat.irhoc2, &task->mat.rhoc1, &task->mat.rhoc2, &task->mat.rhoc3};
+  for (int k = 0; k < 8; ++k) *mp[k] = (real)h.p[3 + k];
+  if (task->stype == ST_SOA) {
+    memcpy(task->grid, buf, 5 * n * sizeof(real));
+  } else {
+    for (int q = 0; q < 5; ++q)
+      for (size_t k = 0; k < n; ++k) task->grid[5 * k + q] = buf[q * n + k];
+  }
+  free(buf);
+  s->host_dirty = true;
+  s->dev_newer = false;
+  return LE2D_OK;
 }
 
 }  // extern "C"

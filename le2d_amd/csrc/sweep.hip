@@ -36,11 +36,24 @@ __device__ __forceinline__ T superbee(T r) {
 
 // Limited flux through the face between upwind node u0 and downwind node u1;
 // um is the node upwind of u0 (simple/le_core.c:50-66 with eps = 1e-6).
+// The reference forms ((phi / 2) * (1 - k)) * (u1 - u0); phi / 2 is exact for
+// every value phi takes (0, 1, 2, r in (1,2), 2r in (0,1)), so
+// (phi / 2) * (1 - k) rounds the same real number as phi * hk with the exact
+// constant hk = (1 - k) / 2, one multiply fewer per face.
+// Slope ratio.  FP64: IEEE division, as the reference.  FP32 tier: the
+// hardware reciprocal (about 1 ulp) times the numerator; that tier is held to
+// an error bound against FP64, not to bit parity, and the correctly rounded
+// FP32 division sequence costs several times more issue slots.
+__device__ __forceinline__ double slope_ratio(double n, double d) { return n / d; }
+__device__ __forceinline__ float slope_ratio(float n, float d) {
+  return n * __builtin_amdgcn_rcpf(d);
+}
+
 template <typename T>
-__device__ __forceinline__ T face_flux(T um, T u0, T u1, T omk) {
+__device__ __forceinline__ T face_flux(T um, T u0, T u1, T hk) {
   const T eps = T(1e-6);
-  const T r = ((u0 - um) + eps) / ((u1 - u0) + eps);
-  return u0 + ((superbee(r) * T(0.5)) * omk) * (u1 - u0);
+  const T r = slope_ratio((u0 - um) + eps, (u1 - u0) + eps);
+  return u0 + (superbee(r) * hk) * (u1 - u0);
 }
 
 // Increment of one invariant from its outflow and inflow face fluxes
@@ -75,10 +88,10 @@ template <typename T>
 __device__ __forceinline__ W4<T> faces(const W4<T> &wm, const W4<T> &w0, const W4<T> &w1,
                                        const W4<T> &w2, const SweepCoef<T> &c) {
   W4<T> f;
-  f.a = face_flux(wm.a, w0.a, w1.a, c.omk1);  // +c1: upwind is the left side
-  f.b = face_flux(w2.b, w1.b, w0.b, c.omk1);  // -c1: upwind is the right side
-  f.c = face_flux(wm.c, w0.c, w1.c, c.omk2);
-  f.d = face_flux(w2.d, w1.d, w0.d, c.omk2);
+  f.a = face_flux(wm.a, w0.a, w1.a, c.hk1);  // +c1: upwind is the left side
+  f.b = face_flux(w2.b, w1.b, w0.b, c.hk1);  // -c1: upwind is the right side
+  f.c = face_flux(wm.c, w0.c, w1.c, c.hk2);
+  f.d = face_flux(w2.d, w1.d, w0.d, c.hk2);
   return f;
 }
 
@@ -310,9 +323,9 @@ __device__ __forceinline__ Node<T> window_xsweep(Node<T> u, W4<T> *sw, W4<T> *sf
 }
 
 template <typename T, int EDGE, int FB>
-__global__ __launch_bounds__(FB, 4) void k_step_xy(const T *__restrict__ src, T *__restrict__ dst,
-                                                int nx, int ny, long pitch, long plane, int seg,
-                                                int nstrips, SweepCoef<T> cx, SweepCoef<T> cy) {
+__global__ __launch_bounds__(FB) void k_step_xy(const T *__restrict__ src, T *__restrict__ dst,
+                                                   int nx, int ny, long pitch, long plane, int seg,
+                                                   int nstrips, SweepCoef<T> cx, SweepCoef<T> cy) {
   __shared__ W4<T> sw[FB];
   __shared__ W4<T> sf[FB];
   const int t = threadIdx.x;
@@ -326,60 +339,77 @@ __global__ __launch_bounds__(FB, 4) void k_step_xy(const T *__restrict__ src, T 
   const int j1 = min(j0 + seg, ny);
 #define ROW(p) load_row_node(src + (long)ypos<EDGE>((p), ny) * pitch, plane, boff)
 
-  // prologue: positions j0-2 .. j0+2; the row of position j0+3 is put in flight
+  // prologue: positions j0-2 .. j0+2 (x-swept), the raw row of j0+3 in flight
   const W4<T> wm2 = inv_y(window_xsweep<T, FB>(ROW(j0 - 2), sw, sf, t, cx), cy);
   W4<T> wm1 = inv_y(window_xsweep<T, FB>(ROW(j0 - 1), sw, sf, t, cx), cy);
-  Node<T> u0 = window_xsweep<T, FB>(ROW(j0), sw, sf, t, cx);
-  Node<T> u1 = window_xsweep<T, FB>(ROW(j0 + 1), sw, sf, t, cx);
-  Node<T> u2 = window_xsweep<T, FB>(ROW(j0 + 2), sw, sf, t, cx);
-  Node<T> next = ROW(j0 + 3);
-  W4<T> w0 = inv_y(u0, cy), w1 = inv_y(u1, cy), w2 = inv_y(u2, cy);
-  W4<T> fin = faces(wm2, wm1, w0, w1, cy);  // face j0-1/2
+  Node<T> A = window_xsweep<T, FB>(ROW(j0), sw, sf, t, cx);
+  Node<T> B = window_xsweep<T, FB>(ROW(j0 + 1), sw, sf, t, cx);
+  Node<T> C = window_xsweep<T, FB>(ROW(j0 + 2), sw, sf, t, cx);
+  Node<T> N0 = ROW(j0 + 3), N1;
+  W4<T> F0 = faces(wm2, wm1, inv_y(A, cy), inv_y(B, cy), cy), F1;  // face j0-1/2
 
-  for (int j = j0; j < j1; ++j) {
-    const Node<T> cur = next;
-    if (j + 2 < j1) next = ROW(j + 4);
-    const W4<T> fout = faces(wm1, w0, w1, w2, cy);  // face j+1/2
-    Node<T> u = u0;
-    apply(u.vy, u.vx, u.syy, u.sxx, u.sxy, w0, fin, fout, cy);
-    if (writer) store_row_node(dst + (long)j * pitch, plane, woff, u);
-    if (j + 1 < j1) {  // uniform over the workgroup
-      const Node<T> un = window_xsweep<T, FB>(cur, sw, sf, t, cx);
-      fin = fout;
-      wm1 = w0;
-      w0 = w1;
-      w1 = w2;
-      u0 = u1;
-      u1 = u2;
-      u2 = un;
-      w2 = inv_y(un, cy);
-    }
+  // Step j.  Slots P0/P1/P2 hold the x-swept nodes of positions j, j+1, j+2
+  // (their y-invariants are re-formed when needed, 6 operations each, rather
+  // than held), FI is the inflow face j-1/2 and NC the raw row of position
+  // j+3.  Row j is written, then position j+3 is x-swept into P0 while NN
+  // receives the raw row of position j+4.  Unrolled six times (slot period 3,
+  // face and prefetch period 2) so no register is copied between steps.
+#define STEP(P0, P1, P2, FI, FO, NC, NN)                                     \
+  {                                                                           \
+    NN = ROW(j + 4);                                                          \
+    const W4<T> w0 = inv_y(P0, cy);                                           \
+    FO = faces(wm1, w0, inv_y(P1, cy), inv_y(P2, cy), cy);                    \
+    apply(P0.vy, P0.vx, P0.syy, P0.sxx, P0.sxy, w0, FI, FO, cy);              \
+    if (writer) store_row_node(dst + (long)j * pitch, plane, woff, P0);      \
+    if (++j >= j1) break;                                                     \
+    wm1.a = w0.a;                                                             \
+    wm1.c = w0.c;                                                             \
+    P0 = window_xsweep<T, FB>(NC, sw, sf, t, cx);                             \
   }
+  int j = j0;
+  for (;;) {
+    STEP(A, B, C, F0, F1, N0, N1)
+    STEP(B, C, A, F1, F0, N1, N0)
+    STEP(C, A, B, F0, F1, N0, N1)
+    STEP(A, B, C, F1, F0, N1, N0)
+    STEP(B, C, A, F0, F1, N0, N1)
+    STEP(C, A, B, F1, F0, N1, N0)
+  }
+#undef STEP
 #undef ROW
 }
 
 // ---------------------------------------------------------------------------
 // Launchers.
 
+template <typename T, int FB>
+static void launch_step_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy, int edge,
+                           int seg, unsigned nblocks, int nstrips) {
+  const T *src = static_cast<const T *>(g->buf[g->cur]);
+  T *dst = static_cast<T *>(g->buf[g->cur ^ 1]);
+  if (edge == EDGE_RING)
+    hipLaunchKernelGGL((k_step_xy<T, EDGE_RING, FB>), dim3(nblocks), dim3(FB), 0, g->stream, src,
+                       dst, g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx, cy);
+  else
+    hipLaunchKernelGGL((k_step_xy<T, EDGE_CLAMP, FB>), dim3(nblocks), dim3(FB), 0, g->stream, src,
+                       dst, g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx, cy);
+}
+
 template <typename T>
 int step_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy, int edge) {
   if (g->elem != (int)sizeof(T)) return set_error(ERR_ARG, "step_xy: element size mismatch");
-  constexpr int FB = 64;
-  const int nstrips = (g->nx + FB - 5) / (FB - 4);
+  const int fb = g->tiles.xy_block == 64 || g->tiles.xy_block == 256 ? g->tiles.xy_block : 128;
+  const int nstrips = (g->nx + fb - 5) / (fb - 4);
   const int seg = max(1, g->tiles.xy_seg);
   const long nsegs = (g->ny + seg - 1) / seg;
   const long nblocks = nsegs * nstrips;
   if (nblocks > 0x7fffffffL) return set_error(ERR_ARG, "step_xy: grid too large");
-  const T *src = static_cast<const T *>(g->buf[g->cur]);
-  T *dst = static_cast<T *>(g->buf[g->cur ^ 1]);
-  if (edge == EDGE_RING)
-    hipLaunchKernelGGL((k_step_xy<T, EDGE_RING, FB>), dim3((unsigned)nblocks), dim3(FB), 0,
-                       g->stream, src, dst, g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx,
-                       cy);
+  if (fb == 64)
+    launch_step_xy<T, 64>(g, cx, cy, edge, seg, (unsigned)nblocks, nstrips);
+  else if (fb == 128)
+    launch_step_xy<T, 128>(g, cx, cy, edge, seg, (unsigned)nblocks, nstrips);
   else
-    hipLaunchKernelGGL((k_step_xy<T, EDGE_CLAMP, FB>), dim3((unsigned)nblocks), dim3(FB), 0,
-                       g->stream, src, dst, g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx,
-                       cy);
+    launch_step_xy<T, 256>(g, cx, cy, edge, seg, (unsigned)nblocks, nstrips);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(ERR_DEVICE, "step_xy launch: %s", hipGetErrorString(e));
   g->cur ^= 1;

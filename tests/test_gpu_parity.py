@@ -62,7 +62,7 @@ def test_fused_step_equals_separate_sweeps():
         res = []
         for fused in (True, False):
             with make(init, 0.3, (1.0, 0.8), (2.0, 1.0, 1.5), stype) as t:
-                t.set_tiles(xy_seg=1 if fused else 0)
+                t.set_tiles(xy_seg=1 if fused else 0, xy_block=64 if fused else 0)
                 for _ in range(3):
                     if fused:
                         t.step()
@@ -141,11 +141,11 @@ def test_tile_shapes_give_identical_bits():
     nx, ny = 1100, 700
     g0 = rng.standard_normal((ny, nx, 5))
     out = []
-    for x_rows, y_seg, y_block, xy_seg in [(1, 7, 64, 1), (4, 64, 128, 5), (16, 1000, 256, 64),
-                                           (3, 33, 256, 1000)]:
+    for shape in [(1, 7, 64, 1, 64), (4, 64, 128, 5, 128), (16, 1000, 256, 64, 256),
+                  (3, 33, 256, 1000, 128), (2, 9, 64, 3, 256)]:
         with make(g0, 0.2, (1.0, 1.0), (2.0, 1.0, 1.5)) as t:
-            t.set_tiles(x_rows, y_seg, y_block, xy_seg)
-            assert t.get_tiles() == (x_rows, y_seg, y_block, xy_seg)
+            t.set_tiles(*shape)
+            assert t.get_tiles() == shape
             t.step_n(2)  # fused step kernel
             for _ in range(2):  # separate sweep kernels
                 t.step_x()
