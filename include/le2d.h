@@ -86,6 +86,50 @@ size_t le2d_device_bytes(le_task *task);
 int le2d_dump(le_task *task, const char *path);
 int le2d_restore(le_task *task, const char *path);
 
+/* ---------------------------------------------------------------------------
+ * Batched instances: many independent grids of uneven sizes and homogeneous
+ * materials, advanced together by one persistent kernel launch per step
+ * (replayed from a hipGraph).  No counterpart in the reference, which steps
+ * one task at a time.  Element type per batch: real_size 4 (FP32 storage and
+ * arithmetic, constants formed in FP64 and rounded once) or 8 (FP64,
+ * bit-identical to stepping each grid alone through le_step).  Edges clamp
+ * as in le_step.  Typical use:
+ *   b = le2d_batch_create(4); le2d_batch_add(b, ...) per instance;
+ *   le2d_batch_commit(b); [le2d_batch_autotune(b, ...)];
+ *   le2d_batch_set_ball / le2d_batch_load per instance;
+ *   le2d_batch_step(b, steps); le2d_batch_read(b, i, host) per instance. */
+typedef struct le2d_batch le2d_batch;
+
+le2d_batch *le2d_batch_create(int real_size);
+void le2d_batch_destroy(le2d_batch *b);
+/* Returns the instance index, or a negative error code. */
+int le2d_batch_add(le2d_batch *b, int nx, int ny, double dt, double hx, double hy, double c1,
+                   double c2, double rho);
+int le2d_batch_count(le2d_batch *b);
+/* Allocates every instance (zeroed fields) and builds the tile table. */
+int le2d_batch_commit(le2d_batch *b);
+size_t le2d_batch_device_bytes(le2d_batch *b);
+/* Disk initial condition on the device, with le_set_ball's arithmetic. */
+int le2d_batch_set_ball(le2d_batch *b, int inst, double cx, double cy, double r, double s);
+/* AOS host arrays of nx*ny nodes of five values of the batch's real type. */
+int le2d_batch_load(le2d_batch *b, int inst, const void *aos);
+int le2d_batch_read(le2d_batch *b, int inst, void *aos);
+/* `steps` steps of every instance: the batched launch (graph replay) or, for
+ * comparison, one plain launch per instance per step. */
+int le2d_batch_step(le2d_batch *b, int steps);
+int le2d_batch_step_plain(le2d_batch *b, int steps);
+int le2d_batch_wait(le2d_batch *b);
+/* Launch shape: window width 64/128/256 lanes and rows per segment. */
+int le2d_batch_set_shape(le2d_batch *b, int fb, int seg);
+int le2d_batch_get_shape(le2d_batch *b, int *fb, int *seg, int *ntiles, int *blocks);
+/* Mean ms per step of `steps` batched (plain = 0) or per-instance (plain = 1)
+ * steps, HIP events on the batch's stream. */
+int le2d_batch_time(le2d_batch *b, int steps, int plain, double *ms);
+/* Times each window width x segment height on the batch and keeps the
+ * fastest (remembered per element size, instance count and node total).
+ * Advances the instances by an even number of steps. */
+int le2d_batch_autotune(le2d_batch *b, int trials, int *best_fb, int *best_seg, double *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

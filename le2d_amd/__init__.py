@@ -39,6 +39,29 @@ _EXT = {
     "le2d_restore": (["T", ctypes.c_char_p], ctypes.c_int),
 }
 
+# include/le2d.h batched-instance API (no le_task argument).
+_VP = ctypes.c_void_p
+_IP = ctypes.POINTER(ctypes.c_int)
+_DP = ctypes.POINTER(ctypes.c_double)
+_BATCH = {
+    "le2d_batch_create": ([ctypes.c_int], _VP),
+    "le2d_batch_destroy": ([_VP], None),
+    "le2d_batch_add": ([_VP, ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 6, ctypes.c_int),
+    "le2d_batch_count": ([_VP], ctypes.c_int),
+    "le2d_batch_commit": ([_VP], ctypes.c_int),
+    "le2d_batch_device_bytes": ([_VP], ctypes.c_size_t),
+    "le2d_batch_set_ball": ([_VP, ctypes.c_int] + [ctypes.c_double] * 4, ctypes.c_int),
+    "le2d_batch_load": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
+    "le2d_batch_read": ([_VP, ctypes.c_int, _VP], ctypes.c_int),
+    "le2d_batch_step": ([_VP, ctypes.c_int], ctypes.c_int),
+    "le2d_batch_step_plain": ([_VP, ctypes.c_int], ctypes.c_int),
+    "le2d_batch_wait": ([_VP], ctypes.c_int),
+    "le2d_batch_set_shape": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "le2d_batch_get_shape": ([_VP, _IP, _IP, _IP, _IP], ctypes.c_int),
+    "le2d_batch_time": ([_VP, ctypes.c_int, ctypes.c_int, _DP], ctypes.c_int),
+    "le2d_batch_autotune": ([_VP, ctypes.c_int, _IP, _IP, _DP], ctypes.c_int),
+}
+
 _libs = {}
 
 
@@ -63,6 +86,10 @@ def load(dtype=np.float64):
     for name, (args, res) in _EXT.items():
         fn = getattr(lib, name)
         fn.argtypes = [PT if a == "T" else a for a in args]
+        fn.restype = res
+    for name, (args, res) in _BATCH.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
         fn.restype = res
     lib.abi = abi
     _libs[dtype] = lib
@@ -195,6 +222,97 @@ class Task:
         if self._open:
             self.lib.le_free_task(self.ref)
             self._open = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Batch:
+    """Many independent grids advanced together (include/le2d.h, le2d_batch_*).
+    State arrays are AOS numpy arrays shaped (ny, nx, 5) of the batch dtype."""
+
+    def __init__(self, dtype=np.float32):
+        self.dtype = np.dtype(dtype)
+        self.lib = load(np.float64)  # the batch API is in both libraries
+        self.h = self.lib.le2d_batch_create(self.dtype.itemsize)
+        if not self.h:
+            raise LE2DError("le2d_batch_create: " + self.lib.le2d_last_error().decode())
+        self.shapes = []
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise LE2DError(f"{what}: {rc} {self.lib.le2d_last_error().decode()}")
+
+    def add(self, nx, ny, dt, h=(1.0, 1.0), mat=(2.0, 1.0, 1.0)):
+        i = self.lib.le2d_batch_add(self.h, nx, ny, dt, h[0], h[1], mat[0], mat[1], mat[2])
+        if i < 0:
+            self._check(-i, "le2d_batch_add")
+        self.shapes.append((ny, nx))
+        return i
+
+    def commit(self):
+        self._check(self.lib.le2d_batch_commit(self.h), "le2d_batch_commit")
+
+    def device_bytes(self):
+        return self.lib.le2d_batch_device_bytes(self.h)
+
+    def set_ball(self, i, c, r, s):
+        self._check(self.lib.le2d_batch_set_ball(self.h, i, c[0], c[1], r, s), "set_ball")
+
+    def load(self, i, state):
+        a = np.ascontiguousarray(state, dtype=self.dtype)
+        assert a.shape == self.shapes[i] + (5,)
+        self._check(self.lib.le2d_batch_load(self.h, i, a.ctypes.data), "le2d_batch_load")
+
+    def read(self, i):
+        a = np.empty(self.shapes[i] + (5,), dtype=self.dtype)
+        self._check(self.lib.le2d_batch_read(self.h, i, a.ctypes.data), "le2d_batch_read")
+        return a
+
+    def step(self, n):
+        self._check(self.lib.le2d_batch_step(self.h, n), "le2d_batch_step")
+
+    def step_plain(self, n):
+        self._check(self.lib.le2d_batch_step_plain(self.h, n), "le2d_batch_step_plain")
+
+    def wait(self):
+        self._check(self.lib.le2d_batch_wait(self.h), "le2d_batch_wait")
+
+    def set_shape(self, fb, seg):
+        self._check(self.lib.le2d_batch_set_shape(self.h, fb, seg), "le2d_batch_set_shape")
+
+    def get_shape(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(self.lib.le2d_batch_get_shape(self.h, *[ctypes.byref(x) for x in v]),
+                    "get_shape")
+        return tuple(x.value for x in v)
+
+    def time(self, steps, plain=False):
+        ms = ctypes.c_double()
+        self._check(self.lib.le2d_batch_time(self.h, steps, int(plain), ctypes.byref(ms)),
+                    "le2d_batch_time")
+        return ms.value
+
+    def autotune(self, trials=3):
+        fb, seg, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._check(self.lib.le2d_batch_autotune(self.h, trials, ctypes.byref(fb),
+                                                 ctypes.byref(seg), ctypes.byref(ms)),
+                    "le2d_batch_autotune")
+        return fb.value, seg.value, ms.value
+
+    def close(self):
+        if self.h:
+            self.lib.le2d_batch_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

@@ -1,0 +1,535 @@
+// batch.hip -- many independent grids of uneven sizes and materials advanced
+// by one persistent launch per step, captured as a hipGraph.
+//
+// Every instance is a ping-pong pair of pitched planes (the single-grid
+// layout) carved from one arena.  A step is one launch of k_step_batch over
+// a table of tiles (instance, strip, row segment) of all instances; each
+// tile runs the fused-step march of sweep_dev.h with its instance's pointers
+// and constants, so a batched instance is bit-identical to the same grid
+// stepped alone (FP64) or agrees to FP32 rounding (FP32).  All instances
+// advance together and share one buffer parity.
+//
+// The launch shape -- window width (64, 128 or 256 lanes) and rows per
+// segment -- is chosen by le2d_batch_autotune, which times candidate shapes
+// on the batch itself and keeps the fastest (per element size, instance
+// count and node total, for the life of the process).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "le2d.h"
+#include "sweep_dev.h"
+
+namespace le2d {
+
+template <typename T>
+struct InstDesc {
+  T *buf[2];
+  int nx, ny;
+  long pitch, plane;
+  SweepCoef<T> cx, cy;
+};
+
+struct TileJob {
+  int inst, strip, j0, j1;
+};
+
+// Persistent over the tile table.  Workgroup b runs on XCD b % 8 (dispatch
+// is round-robin); each XCD walks its own contiguous eighth of the table, so
+// concurrently running workgroups of one XCD process neighbouring strips.
+template <typename T, int FB>
+__global__ __launch_bounds__(FB) void k_step_batch(const InstDesc<T> *__restrict__ inst,
+                                                   const TileJob *__restrict__ tiles, int ntiles,
+                                                   int parity) {
+  __shared__ W4<T> sw[FB];
+  __shared__ W4<T> sf[FB];
+  const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+  const int xcd = blockIdx.x & 7, r = blockIdx.x >> 3;
+  const int lo = (int)((long)ntiles * xcd / 8), hi = (int)((long)ntiles * (xcd + 1) / 8);
+  for (int k = lo + r; k < hi; k += per) {
+    const TileJob tj = tiles[k];
+    const InstDesc<T> &d = inst[tj.inst];
+    march_tile<T, EDGE_CLAMP, FB>(d.buf[parity], d.buf[parity ^ 1], d.nx, d.ny, d.pitch, d.plane,
+                                  tj.strip, tj.j0, tj.j1, d.cx, d.cy, sw, sf);
+  }
+}
+
+// Disk initial condition with the host le_set_ball's arithmetic: the node
+// position and offset in T, the distance as the double square root of the
+// T-rounded sum of squares, strict comparison with r (simple/le_core.c:71-94).
+template <typename T>
+__global__ void k_set_ball(T *__restrict__ buf, int nx, int ny, long pitch, long plane, T hx, T hy,
+                           T cx, T cy, T r, T s) {
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (long)nx * ny) return;
+  const int j = (int)(n / nx), i = (int)(n - (long)j * nx);
+  const T px = hx * (T)i, py = hy * (T)j;
+  const T dx = px - cx, dy = py - cy;
+  const T d2 = dx * dx + dy * dy;
+  if (::sqrt((double)d2) < (double)r) {
+    buf[2 * plane + (long)j * pitch + i] = s;
+    buf[4 * plane + (long)j * pitch + i] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct Inst {
+  int nx, ny;
+  double dt, hx, hy, c1, c2, rho;
+  long pitch, plane;
+  size_t off[2];
+};
+
+struct Shape {
+  int fb, seg;
+};
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, long>, Shape> g_tune;  // (elem, instances, Mnodes) -> shape
+
+// Material constants and sweep constants in double (the reference's
+// le_init_material and le_step_x/y expressions), rounded once to T.
+template <typename T>
+void inst_coef(const Inst &in, SweepCoef<T> *cx, SweepCoef<T> *cy) {
+  const double c1 = in.c1, c2 = in.c2, rho = in.rho;
+  const double irhoc1 = 1.0 / (c1 * rho), irhoc2 = 1.0 / (c2 * rho);
+  const double rhoc1 = c1 * rho, rhoc2 = c2 * rho;
+  const double mu = rho * c2 * c2;
+  const double la = rho * c1 * c1 - 2.0 * mu;
+  const double rhoc3 = rho * c1 * la / (la + 2.0 * mu);
+  for (int a = 0; a < 2; ++a) {
+    const double h = a == 0 ? in.hx : in.hy;
+    const double k1 = in.dt * c1 / h, k2 = in.dt * c2 / h;
+    SweepCoef<T> &c = a == 0 ? *cx : *cy;
+    c.k1 = (T)k1;
+    c.k2 = (T)k2;
+    c.hk1 = (T)((1.0 - k1) * 0.5);
+    c.hk2 = (T)((1.0 - k2) * 0.5);
+    c.irhoc1 = (T)irhoc1;
+    c.irhoc2 = (T)irhoc2;
+    c.rhoc1 = (T)rhoc1;
+    c.rhoc2 = (T)rhoc2;
+    c.rhoc3 = (T)rhoc3;
+  }
+}
+
+int hcheck(hipError_t e, const char *what) {
+  if (e != hipSuccess) return set_error(ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+  return OK;
+}
+
+}  // namespace
+
+struct BatchImpl {
+  int elem = 8;
+  std::vector<Inst> inst;
+  bool committed = false;
+  char *arena = nullptr;
+  size_t arena_bytes = 0;
+  void *d_inst = nullptr;
+  TileJob *d_tiles = nullptr;
+  int ntiles = 0;
+  Shape shape{128, 32};
+  int blocks = 0;
+  int parity = 0;
+  hipStream_t stream = nullptr;
+  // graph[p]: two steps starting from buffer parity p (replays leave p)
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  long steps_done = 0;
+
+  void drop_graphs() {
+    for (int p = 0; p < 2; ++p) {
+      if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
+      if (graph[p]) (void)hipGraphDestroy(graph[p]);
+      exec[p] = nullptr;
+      graph[p] = nullptr;
+    }
+  }
+
+  ~BatchImpl() {
+    drop_graphs();
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (d_tiles) (void)hipFree(d_tiles);
+    if (d_inst) (void)hipFree(d_inst);
+    if (arena) (void)hipFree(arena);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  long total_nodes() const {
+    long n = 0;
+    for (const Inst &in : inst) n += (long)in.nx * in.ny;
+    return n;
+  }
+
+  DevGrid view(int i) {
+    DevGrid g;
+    const Inst &in = inst[i];
+    g.elem = elem;
+    g.nx = in.nx;
+    g.ny = in.ny;
+    g.pitch = in.pitch;
+    g.plane = in.plane;
+    g.buf[0] = arena + in.off[0];
+    g.buf[1] = arena + in.off[1];
+    g.cur = parity;
+    g.stream = stream;
+    g.tiles = default_tiles(elem, in.nx, in.ny);
+    return g;
+  }
+
+  template <typename T>
+  int upload_desc() {
+    std::vector<InstDesc<T>> h(inst.size());
+    for (size_t i = 0; i < inst.size(); ++i) {
+      const Inst &in = inst[i];
+      h[i].buf[0] = reinterpret_cast<T *>(arena + in.off[0]);
+      h[i].buf[1] = reinterpret_cast<T *>(arena + in.off[1]);
+      h[i].nx = in.nx;
+      h[i].ny = in.ny;
+      h[i].pitch = in.pitch;
+      h[i].plane = in.plane;
+      inst_coef<T>(in, &h[i].cx, &h[i].cy);
+    }
+    if (!d_inst && hipMalloc(&d_inst, h.size() * sizeof(InstDesc<T>)) != hipSuccess)
+      return set_error(ERR_NOMEM, "batch: descriptor table");
+    return hcheck(hipMemcpy(d_inst, h.data(), h.size() * sizeof(InstDesc<T>), hipMemcpyHostToDevice),
+                  "batch: descriptors");
+  }
+
+  // Tile table for a shape: instances in order, then segments, then strips,
+  // so neighbouring entries share halo columns.
+  int build_tiles(Shape s) {
+    std::vector<TileJob> h;
+    for (size_t i = 0; i < inst.size(); ++i) {
+      const Inst &in = inst[i];
+      const int strips = (in.nx + s.fb - 5) / (s.fb - 4);
+      const int seg = std::max(1, std::min(s.seg, in.ny));
+      for (int j0 = 0; j0 < in.ny; j0 += seg)
+        for (int st = 0; st < strips; ++st) h.push_back({(int)i, st, j0, std::min(j0 + seg, in.ny)});
+    }
+    if (d_tiles) (void)hipFree(d_tiles);
+    d_tiles = nullptr;
+    if (hipMalloc(&d_tiles, std::max<size_t>(1, h.size()) * sizeof(TileJob)) != hipSuccess)
+      return set_error(ERR_NOMEM, "batch: tile table");
+    int rc = hcheck(hipMemcpy(d_tiles, h.data(), h.size() * sizeof(TileJob), hipMemcpyHostToDevice),
+                    "batch: tiles");
+    if (rc) return rc;
+    ntiles = (int)h.size();
+    shape = s;
+    // persistent grid: resident workgroups on the whole device, a multiple of 8
+    int dev = 0, cus = 256, per_cu = 1;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    occupancy(s.fb, &per_cu);
+    long want = (long)cus * std::max(1, per_cu);
+    want = std::min<long>(want, ((long)ntiles + 7) / 8 * 8);
+    blocks = (int)std::max<long>(8, want / 8 * 8);
+    drop_graphs();
+    return OK;
+  }
+
+  void occupancy(int fb, int *per_cu) {
+    hipError_t e = hipErrorInvalidValue;
+    if (elem == 8) {
+      if (fb == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<double, 64>, 64, 0);
+      if (fb == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<double, 128>, 128, 0);
+      if (fb == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<double, 256>, 256, 0);
+    } else {
+      if (fb == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<float, 64>, 64, 0);
+      if (fb == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<float, 128>, 128, 0);
+      if (fb == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<float, 256>, 256, 0);
+    }
+    if (e != hipSuccess || *per_cu < 1) *per_cu = 1;
+  }
+
+  template <typename T, int FB>
+  void launch_t(int par) {
+    hipLaunchKernelGGL((k_step_batch<T, FB>), dim3(blocks), dim3(FB), 0, stream,
+                       static_cast<const InstDesc<T> *>(d_inst), d_tiles, ntiles, par);
+  }
+
+  void launch(int par) {
+    if (elem == 8) {
+      if (shape.fb == 64) launch_t<double, 64>(par);
+      else if (shape.fb == 128) launch_t<double, 128>(par);
+      else launch_t<double, 256>(par);
+    } else {
+      if (shape.fb == 64) launch_t<float, 64>(par);
+      else if (shape.fb == 128) launch_t<float, 128>(par);
+      else launch_t<float, 256>(par);
+    }
+  }
+
+  // Two steps (parity p then p^1) captured as a graph.
+  int ensure_graph(int p) {
+    if (exec[p]) return OK;
+    int rc = hcheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+    if (rc) return rc;
+    launch(p);
+    launch(p ^ 1);
+    rc = hcheck(hipStreamEndCapture(stream, &graph[p]), "end capture");
+    if (rc) return rc;
+    return hcheck(hipGraphInstantiate(&exec[p], graph[p], nullptr, nullptr, 0), "instantiate");
+  }
+
+  int step(int steps, bool use_graph) {
+    if (!committed) return set_error(ERR_STATE, "batch: not committed");
+    int k = 0;
+    if (use_graph && steps >= 2) {
+      int rc = ensure_graph(parity);
+      if (rc) return rc;
+      for (; k + 1 < steps; k += 2) {
+        rc = hcheck(hipGraphLaunch(exec[parity], stream), "graph launch");
+        if (rc) return rc;
+      }
+    }
+    for (; k < steps; ++k) {
+      launch(parity);
+      parity ^= 1;
+    }
+    steps_done += steps;
+    return hcheck(hipGetLastError(), "batch step");
+  }
+};
+
+}  // namespace le2d
+
+using le2d::BatchImpl;
+
+struct le2d_batch {
+  BatchImpl impl;
+};
+
+extern "C" {
+
+le2d_batch *le2d_batch_create(int real_size) {
+  if (real_size != 4 && real_size != 8) {
+    le2d::set_error(le2d::ERR_ARG, "batch: real_size %d", real_size);
+    return nullptr;
+  }
+  le2d::require_device();
+  le2d_batch *b = new le2d_batch();
+  b->impl.elem = real_size;
+  if (hipStreamCreateWithFlags(&b->impl.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    le2d::set_error(le2d::ERR_DEVICE, "batch: stream");
+    return nullptr;
+  }
+  return b;
+}
+
+void le2d_batch_destroy(le2d_batch *b) { delete b; }
+
+int le2d_batch_add(le2d_batch *b, int nx, int ny, double dt, double hx, double hy, double c1,
+                   double c2, double rho) {
+  if (!b || b->impl.committed) return -le2d::set_error(le2d::ERR_STATE, "batch: add after commit");
+  if (nx < 3 || ny < 3 || !(dt > 0) || !(hx > 0) || !(hy > 0) || !(c1 > 0) || !(c2 > 0) ||
+      !(rho > 0))
+    return -le2d::set_error(le2d::ERR_ARG, "batch: bad instance %dx%d", nx, ny);
+  le2d::Inst in{};
+  in.nx = nx;
+  in.ny = ny;
+  in.dt = dt;
+  in.hx = hx;
+  in.hy = hy;
+  in.c1 = c1;
+  in.c2 = c2;
+  in.rho = rho;
+  b->impl.inst.push_back(in);
+  return (int)b->impl.inst.size() - 1;
+}
+
+int le2d_batch_count(le2d_batch *b) { return b ? (int)b->impl.inst.size() : 0; }
+
+size_t le2d_batch_device_bytes(le2d_batch *b) { return b ? b->impl.arena_bytes : 0; }
+
+int le2d_batch_commit(le2d_batch *b) {
+  if (!b) return le2d::ERR_ARG;
+  BatchImpl &B = b->impl;
+  if (B.committed) return le2d::OK;
+  if (B.inst.empty()) return le2d::set_error(le2d::ERR_ARG, "batch: empty");
+  size_t off = 0;
+  for (auto &in : B.inst) {
+    in.pitch = le2d::plane_pitch(in.nx, B.elem);
+    in.plane = (long)(le2d::grid_bytes(B.elem, in.nx, in.ny) / 5 / B.elem);
+    const size_t bytes = le2d::grid_bytes(B.elem, in.nx, in.ny);
+    for (int k = 0; k < 2; ++k) {
+      in.off[k] = off;
+      off += (bytes + 4095) / 4096 * 4096;
+    }
+  }
+  if (hipMalloc(&B.arena, off) != hipSuccess)
+    return le2d::set_error(le2d::ERR_NOMEM, "batch: arena of %zu bytes", off);
+  B.arena_bytes = off;
+  int rc = le2d::hcheck(hipMemsetAsync(B.arena, 0, off, B.stream), "batch: clear");
+  if (rc) return rc;
+  rc = B.elem == 8 ? B.upload_desc<double>() : B.upload_desc<float>();
+  if (rc) return rc;
+  // default shape until autotuned: the cached winner for this batch, else 128 x 32
+  le2d::Shape s{128, 32};
+  {
+    std::lock_guard<std::mutex> lk(le2d::g_tune_mu);
+    auto it = le2d::g_tune.find({B.elem, (int)B.inst.size(), B.total_nodes() >> 20});
+    if (it != le2d::g_tune.end()) s = it->second;
+  }
+  rc = B.build_tiles(s);
+  if (rc) return rc;
+  B.committed = true;
+  return le2d::hcheck(hipStreamSynchronize(B.stream), "batch: commit");
+}
+
+int le2d_batch_set_ball(le2d_batch *b, int i, double cx, double cy, double r, double s) {
+  if (!b || !b->impl.committed || i < 0 || i >= (int)b->impl.inst.size())
+    return le2d::set_error(le2d::ERR_ARG, "batch: set_ball instance %d", i);
+  BatchImpl &B = b->impl;
+  const le2d::Inst &in = B.inst[i];
+  const long n = (long)in.nx * in.ny;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  void *buf = B.arena + in.off[B.parity];
+  if (B.elem == 8)
+    hipLaunchKernelGGL(le2d::k_set_ball<double>, dim3(blocks), dim3(256), 0, B.stream,
+                       (double *)buf, in.nx, in.ny, in.pitch, in.plane, in.hx, in.hy, cx, cy, r, s);
+  else
+    hipLaunchKernelGGL(le2d::k_set_ball<float>, dim3(blocks), dim3(256), 0, B.stream, (float *)buf,
+                       in.nx, in.ny, in.pitch, in.plane, (float)in.hx, (float)in.hy, (float)cx,
+                       (float)cy, (float)r, (float)s);
+  return le2d::hcheck(hipGetLastError(), "batch: set_ball");
+}
+
+int le2d_batch_load(le2d_batch *b, int i, const void *aos) {
+  if (!b || !b->impl.committed || i < 0 || i >= (int)b->impl.inst.size())
+    return le2d::set_error(le2d::ERR_ARG, "batch: load instance %d", i);
+  le2d::DevGrid g = b->impl.view(i);
+  return le2d::grid_upload(&g, aos, 0);
+}
+
+int le2d_batch_read(le2d_batch *b, int i, void *aos) {
+  if (!b || !b->impl.committed || i < 0 || i >= (int)b->impl.inst.size())
+    return le2d::set_error(le2d::ERR_ARG, "batch: read instance %d", i);
+  int rc = le2d::hcheck(hipStreamSynchronize(b->impl.stream), "batch: read sync");
+  if (rc) return rc;
+  le2d::DevGrid g = b->impl.view(i);
+  return le2d::grid_download(&g, aos, 0);
+}
+
+int le2d_batch_step(le2d_batch *b, int steps) {
+  if (!b || steps < 0) return le2d::set_error(le2d::ERR_ARG, "batch: step %d", steps);
+  return b->impl.step(steps, true);
+}
+
+int le2d_batch_step_plain(le2d_batch *b, int steps) {
+  if (!b || !b->impl.committed || steps < 0)
+    return le2d::set_error(le2d::ERR_ARG, "batch: plain step");
+  BatchImpl &B = b->impl;
+  for (int k = 0; k < steps; ++k) {
+    for (size_t i = 0; i < B.inst.size(); ++i) {
+      le2d::DevGrid g = B.view((int)i);
+      int rc;
+      if (B.elem == 8) {
+        le2d::SweepCoef<double> cx, cy;
+        le2d::inst_coef<double>(B.inst[i], &cx, &cy);
+        rc = le2d::step_xy<double>(&g, cx, cy, le2d::EDGE_CLAMP);
+      } else {
+        le2d::SweepCoef<float> cx, cy;
+        le2d::inst_coef<float>(B.inst[i], &cx, &cy);
+        rc = le2d::step_xy<float>(&g, cx, cy, le2d::EDGE_CLAMP);
+      }
+      if (rc) return rc;
+    }
+    B.parity ^= 1;
+  }
+  B.steps_done += steps;
+  return le2d::OK;
+}
+
+int le2d_batch_wait(le2d_batch *b) {
+  if (!b) return le2d::ERR_ARG;
+  return le2d::hcheck(hipStreamSynchronize(b->impl.stream), "batch: wait");
+}
+
+int le2d_batch_set_shape(le2d_batch *b, int fb, int seg) {
+  if (!b || !b->impl.committed || (fb != 64 && fb != 128 && fb != 256) || seg < 1)
+    return le2d::set_error(le2d::ERR_ARG, "batch: shape %d %d", fb, seg);
+  return b->impl.build_tiles({fb, seg});
+}
+
+int le2d_batch_get_shape(le2d_batch *b, int *fb, int *seg, int *ntiles, int *blocks) {
+  if (!b) return le2d::ERR_ARG;
+  *fb = b->impl.shape.fb;
+  *seg = b->impl.shape.seg;
+  *ntiles = b->impl.ntiles;
+  *blocks = b->impl.blocks;
+  return le2d::OK;
+}
+
+// Time `steps` steps of the batched launch (graph) or the per-instance loop;
+// mean ms per step, HIP events on the batch stream.
+int le2d_batch_time(le2d_batch *b, int steps, int plain, double *ms) {
+  if (!b || steps < 1) return le2d::ERR_ARG;
+  BatchImpl &B = b->impl;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, B.stream);
+  int rc = plain ? le2d_batch_step_plain(b, steps) : le2d_batch_step(b, steps);
+  (void)hipEventRecord(e1, B.stream);
+  hipError_t e = hipEventSynchronize(e1);
+  float t = 0;
+  (void)hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (e != hipSuccess) return le2d::set_error(le2d::ERR_DEVICE, "batch time: %s", hipGetErrorString(e));
+  *ms = t / steps;
+  return le2d::OK;
+}
+
+// Times every candidate shape for `trials` steps (after one warm-up step) and
+// keeps the fastest.  Stepping the batch is part of tuning: the instances
+// advance by (1 + trials) steps per candidate, an even number in total, so
+// callers tune before loading their initial state or account for it.
+int le2d_batch_autotune(le2d_batch *b, int trials, int *best_fb, int *best_seg, double *best_ms) {
+  if (!b || !b->impl.committed || trials < 1) return le2d::ERR_ARG;
+  BatchImpl &B = b->impl;
+  const int fbs[3] = {64, 128, 256};
+  const int segs[4] = {16, 32, 64, 128};
+  le2d::Shape best{B.shape};
+  double best_t = 1e30;
+  for (int fb : fbs) {
+    for (int seg : segs) {
+      int rc = B.build_tiles({fb, seg});
+      if (rc) return rc;
+      rc = B.step(2, true);  // captures the graph, warms up
+      if (rc) return rc;
+      double ms = 0;
+      rc = le2d_batch_time(b, 2 * trials, 0, &ms);
+      if (rc) return rc;
+      if (ms < best_t) {
+        best_t = ms;
+        best = {fb, seg};
+      }
+    }
+  }
+  int rc = B.build_tiles(best);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(le2d::g_tune_mu);
+    le2d::g_tune[{B.elem, (int)B.inst.size(), B.total_nodes() >> 20}] = best;
+  }
+  if (best_fb) *best_fb = best.fb;
+  if (best_seg) *best_seg = best.seg;
+  if (best_ms) *best_ms = best_t;
+  return le2d::OK;
+}
+
+}  // extern "C"

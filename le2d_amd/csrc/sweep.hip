@@ -1,186 +1,8 @@
-// sweep.hip -- the x- and y-sweep kernels for gfx950 and their launchers.
-//
-// Numerics (reference simple/le_core.c:29-69, :192-257): per sweep each node's
-// state u = (vx, vy, sxx, sxy, syy) is projected on four Riemann invariants
-// with speeds +c1, -c1, +c2, -c2; each invariant is advanced by the
-// superbee-limited second-order TVD scheme with clamped edges; the increment
-// is projected back.  All invariants are formed from pre-sweep values.
-//
-// Face formulation.  The reference evaluates, per node and invariant, the two
-// face fluxes f(i+1/2) and f(i-1/2) (tvd2, :48-69).  The flux a node computes
-// for its right face is, operand for operand, the one its right neighbour
-// computes for its left face, so each face is evaluated once here and shared:
-// one division, one limiter and one flux per face and invariant instead of
-// two.  The node update then applies u - k*(f_out - f_in) - u in the
-// reference's order, which keeps FP64 results bit-identical when the
-// compiler does not contract multiply-adds (-ffp-contract=off).
-//
-// Layout: five planes, element (i, j) of plane q at q*plane + j*pitch + i.
-#include "engine.h"
+// sweep.hip -- the x-sweep, y-sweep and fused-step kernels for gfx950 and
+// their launchers.  Numerics and the fused-step tile march: sweep_dev.h.
+#include "sweep_dev.h"
 
 namespace le2d {
-
-// ---------------------------------------------------------------------------
-// Scalar pieces.
-
-// Superbee limiter with the reference's ternary min/max (simple/le_core.c:31-38):
-// max(0, max(min(1, 2r), min(2, r))).
-template <typename T>
-__device__ __forceinline__ T superbee(T r) {
-  const T r2 = T(2) * r;
-  const T a = (T(1) > r2) ? r2 : T(1);
-  const T b = (T(2) > r) ? r : T(2);
-  const T m = (a > b) ? a : b;
-  return (T(0) > m) ? T(0) : m;
-}
-
-// Limited flux through the face between upwind node u0 and downwind node u1;
-// um is the node upwind of u0 (simple/le_core.c:50-66 with eps = 1e-6).
-// The reference forms ((phi / 2) * (1 - k)) * (u1 - u0); phi / 2 is exact for
-// every value phi takes (0, 1, 2, r in (1,2), 2r in (0,1)), so
-// (phi / 2) * (1 - k) rounds the same real number as phi * hk with the exact
-// constant hk = (1 - k) / 2, one multiply fewer per face.
-// Slope ratio.  FP64: IEEE division, as the reference.  FP32 tier: the
-// hardware reciprocal (about 1 ulp) times the numerator; that tier is held to
-// an error bound against FP64, not to bit parity, and the correctly rounded
-// FP32 division sequence costs several times more issue slots.
-__device__ __forceinline__ double slope_ratio(double n, double d) { return n / d; }
-__device__ __forceinline__ float slope_ratio(float n, float d) {
-  return n * __builtin_amdgcn_rcpf(d);
-}
-
-template <typename T>
-__device__ __forceinline__ T face_flux(T um, T u0, T u1, T hk) {
-  const T eps = T(1e-6);
-  const T r = slope_ratio((u0 - um) + eps, (u1 - u0) + eps);
-  return u0 + (superbee(r) * hk) * (u1 - u0);
-}
-
-// Increment of one invariant from its outflow and inflow face fluxes
-// (simple/le_core.c:68 and :253-256): (u - k*(out - in)) - u.
-template <typename T>
-__device__ __forceinline__ T face_delta(T u, T k, T out, T in) {
-  return (u - k * (out - in)) - u;
-}
-
-template <typename T>
-struct W4 {
-  T a, b, c, d;  // speeds +c1, -c1, +c2, -c2
-};
-
-// Invariants for a sweep along the axis whose velocity is vn and whose normal
-// stress is sn (simple/le_core.c:192-218).
-template <typename T>
-__device__ __forceinline__ W4<T> invariants(T vn, T sn, T vt, T sxy, const SweepCoef<T> &c) {
-  const T n = sn * c.irhoc1;
-  const T m = sxy * c.irhoc2;
-  W4<T> w;
-  w.a = vn - n;
-  w.b = vn + n;
-  w.c = vt - m;
-  w.d = vt + m;
-  return w;
-}
-
-// Faces of the four invariants between positions p and p+1, given the
-// invariants at p-1, p, p+1, p+2.
-template <typename T>
-__device__ __forceinline__ W4<T> faces(const W4<T> &wm, const W4<T> &w0, const W4<T> &w1,
-                                       const W4<T> &w2, const SweepCoef<T> &c) {
-  W4<T> f;
-  f.a = face_flux(wm.a, w0.a, w1.a, c.hk1);  // +c1: upwind is the left side
-  f.b = face_flux(w2.b, w1.b, w0.b, c.hk1);  // -c1: upwind is the right side
-  f.c = face_flux(wm.c, w0.c, w1.c, c.hk2);
-  f.d = face_flux(w2.d, w1.d, w0.d, c.hk2);
-  return f;
-}
-
-// Back-transform of the increments (simple/le_core.c:221-249).  vn/sn are
-// the along-axis velocity and normal stress, vt/so the tangential velocity
-// and the other normal stress.
-template <typename T>
-__device__ __forceinline__ void apply(T &vn, T &vt, T &sn, T &so, T &sxy, const W4<T> &w,
-                                      const W4<T> &fl, const W4<T> &fr,
-                                      const SweepCoef<T> &c) {
-  const T e1 = T(0.5) * face_delta(w.a, c.k1, fr.a, fl.a);
-  const T e2 = T(0.5) * face_delta(w.b, c.k1, fl.b, fr.b);
-  const T e3 = T(0.5) * face_delta(w.c, c.k2, fr.c, fl.c);
-  const T e4 = T(0.5) * face_delta(w.d, c.k2, fl.d, fr.d);
-  vn = vn + (e1 + e2);
-  vt = vt + (e3 + e4);
-  sn = sn + (e2 - e1) * c.rhoc1;
-  so = so + (e2 - e1) * c.rhoc3;
-  sxy = sxy + c.rhoc2 * (e4 - e3);
-}
-
-// ---------------------------------------------------------------------------
-// Rows and nodes.
-
-// Row of window position p along y: clamp (le_step) or the ring edge
-// (le_step_soa / le_step_y_cf).
-template <int EDGE>
-__device__ __forceinline__ int ypos(int p, int ny) {
-  if (p < 0) return 0;
-  if (p < ny) return p;
-  return EDGE == EDGE_CLAMP ? ny - 1 : max(p - 5, 0);
-}
-
-template <typename T>
-struct Node {
-  T vx, vy, sxx, sxy, syy;
-};
-
-template <typename T>
-__device__ __forceinline__ Node<T> load_node(const T *__restrict__ p, long plane) {
-  Node<T> n;
-  n.vx = p[0];
-  n.vy = p[plane];
-  n.sxx = p[2 * plane];
-  n.sxy = p[3 * plane];
-  n.syy = p[4 * plane];
-  return n;
-}
-
-// Loads and stores through a wavefront-uniform row base (kept in SGPRs) and a
-// 32-bit per-lane byte offset, which maps to the SGPR-base + VGPR-offset
-// addressing of global memory instructions and keeps VGPRs free of 64-bit
-// addresses.  Planes are < 4 GiB, so the offset always fits.
-template <typename T>
-__device__ __forceinline__ T ld_u(const T *__restrict__ base, unsigned boff) {
-  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
-}
-
-template <typename T>
-__device__ __forceinline__ void st_u(T *__restrict__ base, unsigned boff, T v) {
-  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
-}
-
-template <typename T>
-__device__ __forceinline__ Node<T> load_row_node(const T *__restrict__ row, long plane,
-                                                 unsigned boff) {
-  Node<T> n;
-  n.vx = ld_u(row, boff);
-  n.vy = ld_u(row + plane, boff);
-  n.sxx = ld_u(row + 2 * plane, boff);
-  n.sxy = ld_u(row + 3 * plane, boff);
-  n.syy = ld_u(row + 4 * plane, boff);
-  return n;
-}
-
-template <typename T>
-__device__ __forceinline__ void store_row_node(T *__restrict__ row, long plane, unsigned boff,
-                                               const Node<T> &u) {
-  st_u(row, boff, u.vx);
-  st_u(row + plane, boff, u.vy);
-  st_u(row + 2 * plane, boff, u.sxx);
-  st_u(row + 3 * plane, boff, u.sxy);
-  st_u(row + 4 * plane, boff, u.syy);
-}
-
-template <typename T>
-__device__ __forceinline__ W4<T> inv_y(const Node<T> &n, const SweepCoef<T> &c) {
-  return invariants(n.vy, n.syy, n.vx, n.sxy, c);
-}
 
 // ---------------------------------------------------------------------------
 // x-sweep.  A workgroup of BX lanes owns BX consecutive nodes of a row and
@@ -275,108 +97,18 @@ __global__ __launch_bounds__(256) void k_sweep_y(const T *__restrict__ src, T *_
 }
 
 // ---------------------------------------------------------------------------
-// Fused step: the x-sweep and the y-sweep of one time step in one pass over
-// HBM (5 reads + 5 writes per node-update instead of twice that).
-//
-// A workgroup of FB lanes owns a window of FB consecutive columns starting two
-// columns left of its strip of FB-4 output columns, and a segment of `seg`
-// rows.  Lane t reads column x0-2+t (clamped into the grid), so the two halo
-// columns on each side of the strip come from ordinary lanes: no extra loads,
-// no divergent work.  Walking positions p = j0-2 .. j1+1 down the strip, each
-// position's row (ypos-mapped) is read once and x-swept through LDS (every
-// face of the strip in one pass over lanes 2..FB-2); each lane's post-x node
-// then feeds the register y-march of k_sweep_y.  Only rows [j0, j1) of the
-// FB-4 inner columns are written.  The y-march consumes x-swept values of the
-// rows just outside the segment, recomputed here from the unswept input,
-// which is exactly what the y-sweep of the two-kernel sequence reads.
-//
-// Workgroups are renumbered so that the tiles one XCD receives (the hardware
-// deals workgroups to the 8 XCDs round-robin) are consecutive strips of the
-// same segments: strips that share halo columns share an L2.
-__device__ __forceinline__ int xcd_tile(int L, int N) {
-  const int per = N >> 3, rem = N & 7, xcd = L & 7, k = L >> 3;
-  return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
-}
-
-template <typename T>
-__device__ __forceinline__ void store_node(T *p, long plane, const Node<T> &u) {
-  p[0] = u.vx;
-  p[plane] = u.vy;
-  p[2 * plane] = u.sxx;
-  p[3 * plane] = u.sxy;
-  p[4 * plane] = u.syy;
-}
-
-// x-sweep of one window row; returns the lane's post-x node (meaningful on
-// lanes 2..FB-3).  sw[t] holds lane t's invariants, sf[t] the face between
-// lanes t-1 and t.
-template <typename T, int FB>
-__device__ __forceinline__ Node<T> window_xsweep(Node<T> u, W4<T> *sw, W4<T> *sf, int t,
-                                                 const SweepCoef<T> &c) {
-  const W4<T> w = invariants(u.vx, u.sxx, u.vy, u.sxy, c);
-  sw[t] = w;
-  __syncthreads();
-  if (t >= 2 && t <= FB - 2) sf[t] = faces(sw[t - 2], sw[t - 1], sw[t], sw[t + 1], c);
-  __syncthreads();
-  if (t >= 2 && t <= FB - 3) apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, sf[t], sf[t + 1], c);
-  return u;
-}
-
+// Fused step over one grid (layout and march: march_tile in sweep_dev.h).
 template <typename T, int EDGE, int FB>
 __global__ __launch_bounds__(FB) void k_step_xy(const T *__restrict__ src, T *__restrict__ dst,
-                                                   int nx, int ny, long pitch, long plane, int seg,
-                                                   int nstrips, SweepCoef<T> cx, SweepCoef<T> cy) {
+                                                int nx, int ny, long pitch, long plane, int seg,
+                                                int nstrips, SweepCoef<T> cx, SweepCoef<T> cy) {
   __shared__ W4<T> sw[FB];
   __shared__ W4<T> sf[FB];
-  const int t = threadIdx.x;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int strip = tile % nstrips, segi = tile / nstrips;
-  const int x = strip * (FB - 4) - 2 + t;
-  const unsigned boff = (unsigned)min(max(x, 0), nx - 1) * (unsigned)sizeof(T);
-  const unsigned woff = (unsigned)max(x, 0) * (unsigned)sizeof(T);
-  const bool writer = t >= 2 && t <= FB - 3 && x < nx;
   const int j0 = segi * seg;
   const int j1 = min(j0 + seg, ny);
-#define ROW(p) load_row_node(src + (long)ypos<EDGE>((p), ny) * pitch, plane, boff)
-
-  // prologue: positions j0-2 .. j0+2 (x-swept), the raw row of j0+3 in flight
-  const W4<T> wm2 = inv_y(window_xsweep<T, FB>(ROW(j0 - 2), sw, sf, t, cx), cy);
-  W4<T> wm1 = inv_y(window_xsweep<T, FB>(ROW(j0 - 1), sw, sf, t, cx), cy);
-  Node<T> A = window_xsweep<T, FB>(ROW(j0), sw, sf, t, cx);
-  Node<T> B = window_xsweep<T, FB>(ROW(j0 + 1), sw, sf, t, cx);
-  Node<T> C = window_xsweep<T, FB>(ROW(j0 + 2), sw, sf, t, cx);
-  Node<T> N0 = ROW(j0 + 3), N1;
-  W4<T> F0 = faces(wm2, wm1, inv_y(A, cy), inv_y(B, cy), cy), F1;  // face j0-1/2
-
-  // Step j.  Slots P0/P1/P2 hold the x-swept nodes of positions j, j+1, j+2
-  // (their y-invariants are re-formed when needed, 6 operations each, rather
-  // than held), FI is the inflow face j-1/2 and NC the raw row of position
-  // j+3.  Row j is written, then position j+3 is x-swept into P0 while NN
-  // receives the raw row of position j+4.  Unrolled six times (slot period 3,
-  // face and prefetch period 2) so no register is copied between steps.
-#define STEP(P0, P1, P2, FI, FO, NC, NN)                                     \
-  {                                                                           \
-    NN = ROW(j + 4);                                                          \
-    const W4<T> w0 = inv_y(P0, cy);                                           \
-    FO = faces(wm1, w0, inv_y(P1, cy), inv_y(P2, cy), cy);                    \
-    apply(P0.vy, P0.vx, P0.syy, P0.sxx, P0.sxy, w0, FI, FO, cy);              \
-    if (writer) store_row_node(dst + (long)j * pitch, plane, woff, P0);      \
-    if (++j >= j1) break;                                                     \
-    wm1.a = w0.a;                                                             \
-    wm1.c = w0.c;                                                             \
-    P0 = window_xsweep<T, FB>(NC, sw, sf, t, cx);                             \
-  }
-  int j = j0;
-  for (;;) {
-    STEP(A, B, C, F0, F1, N0, N1)
-    STEP(B, C, A, F1, F0, N1, N0)
-    STEP(C, A, B, F0, F1, N0, N1)
-    STEP(A, B, C, F1, F0, N1, N0)
-    STEP(B, C, A, F0, F1, N0, N1)
-    STEP(C, A, B, F1, F0, N1, N0)
-  }
-#undef STEP
-#undef ROW
+  march_tile<T, EDGE, FB>(src, dst, nx, ny, pitch, plane, strip, j0, j1, cx, cy, sw, sf);
 }
 
 // ---------------------------------------------------------------------------

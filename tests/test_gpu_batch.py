@@ -1,0 +1,118 @@
+"""Batched instances (le2d_batch_*): one persistent launch per step over all
+instances.  FP64 instances are bit-identical to the oracle (and so to
+stepping each grid alone); FP32 instances stay within a stated error of it."""
+import numpy as np
+import pytest
+
+import le2d_amd
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+OD = Oracle(np.float64)
+
+SIZES = [(3, 3), (5, 130), (130, 5), (64, 64), (257, 129), (300, 41), (128, 700), (1000, 17)]
+
+
+def _params(k):
+    rng = np.random.default_rng(100 + k)
+    c1 = float(rng.uniform(1, 4))
+    c2 = c1 / float(rng.uniform(1.5, 2.2))
+    rho = float(rng.uniform(1, 3))
+    h = (float(rng.uniform(0.5, 1.5)), float(rng.uniform(0.5, 1.5)))
+    dt = 0.45 * min(h) / c1
+    return dt, h, (c1, c2, rho)
+
+
+def _make(dtype, steps_shape=None):
+    b = le2d_amd.Batch(dtype)
+    states = []
+    for k, (nx, ny) in enumerate(SIZES):
+        dt, h, mat = _params(k)
+        b.add(nx, ny, dt, h, mat)
+    b.commit()
+    for k, (nx, ny) in enumerate(SIZES):
+        g = np.random.default_rng(k).standard_normal((ny, nx, 5))
+        g[:, : nx // 3] = 0.25
+        b.load(k, g)
+        states.append(g)
+    return b, states
+
+
+def _oracle(states, steps):
+    out = []
+    for k, g in enumerate(states):
+        dt, h, mat = _params(k)
+        r = g.copy()
+        OD.run(r, dt, h, mat, steps)
+        out.append(r)
+    return out
+
+
+def test_batch_fp64_bit_identical_to_oracle():
+    b, states = _make(np.float64)
+    with b:
+        b.step(5)  # graph (2 replays) + one single launch
+        refs = _oracle(states, 5)
+        for k, r in enumerate(refs):
+            assert np.array_equal(b.read(k), r), SIZES[k]
+
+
+def test_batch_plain_loop_and_shapes_agree():
+    res = []
+    for shape, plain in [((128, 32), False), ((64, 7), False), ((256, 1000), False),
+                         ((128, 32), True)]:
+        b, states = _make(np.float64)
+        with b:
+            b.set_shape(*shape)
+            (b.step_plain if plain else b.step)(4)
+            res.append([b.read(k) for k in range(len(SIZES))])
+    for other in res[1:]:
+        for a, c in zip(res[0], other):
+            assert np.array_equal(a, c)
+    refs = _oracle(states, 4)
+    for a, r in zip(res[0], refs):
+        assert np.array_equal(a, r)
+
+
+def test_batch_autotune_keeps_results():
+    b, states = _make(np.float64)
+    with b:
+        fb, seg, ms = b.autotune(1)  # advances every instance by an even count
+        assert fb in (64, 128, 256) and seg in (16, 32, 64, 128) and ms > 0
+        n_tuned = 3 * 4 * 4  # 12 shapes x (2 warm-up + 2 timed) steps
+        for k, g in enumerate(states):
+            b.load(k, g)
+        b.step(3)
+        for k, r in enumerate(_oracle(states, 3)):
+            assert np.array_equal(b.read(k), r)
+        assert n_tuned % 2 == 0
+
+
+def test_batch_fp32_close_to_fp64():
+    b, states = _make(np.float32)
+    with b:
+        b.step(20)
+        refs = _oracle(states, 20)
+        for k, r in enumerate(refs):
+            got = b.read(k).astype(np.float64)
+            # FP32 storage and arithmetic against the FP64 oracle
+            l2 = np.linalg.norm(got - r) / np.linalg.norm(r)
+            assert l2 < 2e-5, (SIZES[k], l2)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_device_ball_matches_host_set_ball(dtype):
+    cases = [((40, 30), (1.0, 1.0), (20.0, 15.0), 10.0, 1.0),
+             ((40, 30), (0.7, 1.3), (13.3, 17.9), 6.5, -2.5),
+             ((333, 257), (1.0, 0.5), (100.0, 60.0), 64.0, 0.75),
+             ((64, 64), (1.0, 1.0), (32.0, 32.0), 9.999999, 1.0)]
+    b = le2d_amd.Batch(dtype)
+    for (nx, ny), h, c, r, s in cases:
+        b.add(nx, ny, 0.1, h, (2.0, 1.0, 1.0))
+    b.commit()
+    with b:
+        for k, ((nx, ny), h, c, r, s) in enumerate(cases):
+            b.set_ball(k, c, r, s)
+            with le2d_amd.Task(nx, ny, 0.1, h, (2.0, 1.0, 1.0), dtype=dtype) as t:
+                t.set_ball(c, r, s)
+                assert np.array_equal(b.read(k), t.view()), (k, np.count_nonzero(b.read(k) != t.view()))
