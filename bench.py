@@ -81,6 +81,83 @@ def measured_traffic(n, dtype):
     return json.load(open(p)).get("hbm_bytes_per_launch")
 
 
+def mixed_instances(count=256, seed=3):
+    """Mixed-size workload: sides log-uniform in [128, 2048] (independent),
+    c1 ~ U[1,4], c2 = c1/U[1.5,2.2], rho ~ U[1,3], dt = 0.5*h/c1 with h = 1,
+    disk radius U[4,32] inside the grid, sxx = syy = 1."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        nx, ny = (int(round(np.exp(rng.uniform(np.log(128), np.log(2048))))) for _ in range(2))
+        c1 = float(rng.uniform(1.0, 4.0))
+        c2 = c1 / float(rng.uniform(1.5, 2.2))
+        rho = float(rng.uniform(1.0, 3.0))
+        r = float(rng.uniform(4.0, 32.0))
+        cx, cy = float(rng.uniform(r, nx - r)), float(rng.uniform(r, ny - r))
+        out.append(dict(nx=nx, ny=ny, c1=c1, c2=c2, rho=rho, dt=0.5 / c1, r=r, cx=cx, cy=cy))
+    return out
+
+
+def run_batch(le2d_amd, dtype, insts, steps, plain_steps):
+    """Batched launch (autotuned, graph-replayed) vs one plain launch per
+    instance per step, same instances.  Returns (result dict, batch)."""
+    b = le2d_amd.Batch(dtype)
+    for p in insts:
+        b.add(p["nx"], p["ny"], p["dt"], (1.0, 1.0), (p["c1"], p["c2"], p["rho"]))
+    b.commit()
+    fb, seg, _ = b.autotune(2)  # on the zero state, which stays exactly zero
+    for i, p in enumerate(insts):
+        b.set_ball(i, (p["cx"], p["cy"]), p["r"], 1.0)
+    nodes = sum(p["nx"] * p["ny"] for p in insts)
+    ms_batch = b.time(steps)
+    ms_plain = b.time(plain_steps, plain=True)
+    b.wait()
+    es = np.dtype(dtype).itemsize
+    res = {"instances": len(insts), "nodes_per_step": nodes, "steps": steps,
+           "shape": {"window_lanes": fb, "segment_rows": seg},
+           "batched_Gnode_per_s": round(nodes / ms_batch / 1e6, 3),
+           "plain_Gnode_per_s": round(nodes / ms_plain / 1e6, 3),
+           "speedup_vs_plain": round(ms_plain / ms_batch, 3),
+           "ms_per_step_batched": round(ms_batch, 4), "ms_per_step_plain": round(ms_plain, 4),
+           "plain_steps_timed": plain_steps,
+           "achieved_GBps_2sweep": round(nodes * 20 * es / ms_batch / 1e6, 1)}
+    return res, b
+
+
+def field_errors(a, ref):
+    """Relative L2 and Linf of `a` against `ref` over all fields (FP64 sums)."""
+    d = a.astype(np.float64) - ref
+    return (float(np.sqrt(np.sum(d * d))), float(np.sqrt(np.sum(ref * ref))),
+            float(np.max(np.abs(d))), float(np.max(np.abs(ref))))
+
+
+def batch_report(le2d_amd, steps=500, plain_steps=50, count=256):
+    insts = mixed_instances(count)
+    out = {"workload": f"{count} instances, sides log-uniform [128, 2048], per-instance material "
+                       f"and disk, {steps} steps"}
+    r64, b64 = run_batch(le2d_amd, np.float64, insts, steps, plain_steps)
+    r32, b32 = run_batch(le2d_amd, np.float32, insts, steps, plain_steps)
+    # both batches advanced steps + plain_steps (+1 warm-up inside time()) from the disk
+    e2n = e2d = einf = rinf = 0.0
+    worst = 0.0
+    for i in range(len(insts)):
+        ref = b64.read(i)
+        a = b32.read(i)
+        n2, d2, ni, di = field_errors(a, ref)
+        e2n += n2 * n2
+        e2d += d2 * d2
+        einf, rinf = max(einf, ni), max(rinf, di)
+        worst = max(worst, n2 / d2 if d2 > 0 else 0.0)
+    b64.close()
+    b32.close()
+    r32["l2_rel_vs_fp64"] = float(np.sqrt(e2n / e2d))
+    r32["l2_rel_vs_fp64_worst_instance"] = worst
+    r32["linf_rel_vs_fp64"] = einf / rinf
+    out["fp32"] = r32
+    out["fp64"] = r64
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +168,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="steps timed per sweep with HIP events for the roofline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch leg")
+    ap.add_argument("--batch-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -137,6 +216,11 @@ def main():
     copy_ms = le2d_amd.copy_probe(nodes * 5 * es, 10)
     copy_gbs = 2 * nodes * 5 * es / copy_ms / 1e6
 
+    batched = None
+    if not args.no_batch:
+        print("[bench] mixed-size batch", file=sys.stderr, flush=True)
+        batched = batch_report(le2d_amd, steps=args.batch_steps)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
@@ -156,6 +240,7 @@ def main():
         "hbm_copy_GBps": round(copy_gbs, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out))
 

@@ -86,6 +86,40 @@ size_t le2d_device_bytes(le_task *task);
 int le2d_dump(le_task *task, const char *path);
 int le2d_restore(le_task *task, const char *path);
 
+/* Heterogeneous media: `count` materials (c1, c2, rho triples, doubles) and
+ * one material index per node (ids[i + j*nx] < count; NULL = all material 0).
+ * From then on le_step, le_step_n, le_step_soa and le_step_cf update every
+ * node entirely in its own material: the stencil of a node is projected on
+ * that node's eigenvectors, advected with its Courant numbers (dt*c/h) and
+ * projected back with its constants.  The reference has one material per
+ * task and defines no interface rule; with all ids equal this is the
+ * reference scheme exactly (bit-identical in FP64).  The constants are formed
+ * in FP64 and rounded once to `real`; task->mat is not used for stepping
+ * while a plane is set.  count = 0 removes the plane.  Single sweeps
+ * (le_step_x, le_step_y, ...) are refused while a plane is set. */
+int le2d_set_material_plane(le_task *task, int count, const double *c1_c2_rho,
+                            const unsigned char *ids);
+/* Total elastic energy in FP64: the sum over nodes of
+ * rho/2 |v|^2 + (sxx+syy)^2 / (8 (lambda+mu)) + ((sxx-syy)^2/2 + 2 sxy^2) / (4 mu),
+ * times h.x*h.y, with mu = rho c2^2 and lambda = rho c1^2 - 2 mu of each
+ * node's material (task->mat without a plane). */
+int le2d_energy(le_task *task, double *e);
+
+/* A read-only view of a device state: the current buffer, element size (4 or
+ * 8), grid size, row pitch and plane stride in elements.  Valid until the
+ * next step or free of its owner.  Views of tasks from liblecore_d.so and
+ * liblecore_f.so (and of batch instances) can be compared with each other. */
+typedef struct {
+  const void *base;
+  int elem, nx, ny;
+  long pitch, plane;
+} le2d_view;
+
+int le2d_task_view(le_task *task, le2d_view *v);
+/* out = {||a - ref||_2, ||ref||_2, max|a - ref|, max|ref|} over all five
+ * fields, accumulated in FP64 on the device. */
+int le2d_diff_norms(const le2d_view *a, const le2d_view *ref, double out[4]);
+
 /* ---------------------------------------------------------------------------
  * Batched instances: many independent grids of uneven sizes and homogeneous
  * materials, advanced together by one persistent kernel launch per step
@@ -129,6 +163,8 @@ int le2d_batch_time(le2d_batch *b, int steps, int plain, double *ms);
  * fastest (remembered per element size, instance count and node total).
  * Advances the instances by an even number of steps. */
 int le2d_batch_autotune(le2d_batch *b, int trials, int *best_fb, int *best_seg, double *best_ms);
+/* View of instance `inst`'s current state (see le2d_view). */
+int le2d_batch_view(le2d_batch *b, int inst, le2d_view *v);
 
 #ifdef __cplusplus
 }

@@ -36,8 +36,17 @@ _EXT = {
                         ctypes.c_int),
     "le2d_device_bytes": (["T"], ctypes.c_size_t),
     "le2d_dump": (["T", ctypes.c_char_p], ctypes.c_int),
+    "le2d_set_material_plane": (["T", ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
+                                ctypes.c_int),
+    "le2d_energy": (["T", ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "le2d_task_view": (["T", ctypes.c_void_p], ctypes.c_int),
     "le2d_restore": (["T", ctypes.c_char_p], ctypes.c_int),
 }
+
+class le2d_view(ctypes.Structure):
+    _fields_ = [("base", ctypes.c_void_p), ("elem", ctypes.c_int), ("nx", ctypes.c_int),
+                ("ny", ctypes.c_int), ("pitch", ctypes.c_long), ("plane", ctypes.c_long)]
+
 
 # include/le2d.h batched-instance API (no le_task argument).
 _VP = ctypes.c_void_p
@@ -60,6 +69,9 @@ _BATCH = {
     "le2d_batch_get_shape": ([_VP, _IP, _IP, _IP, _IP], ctypes.c_int),
     "le2d_batch_time": ([_VP, ctypes.c_int, ctypes.c_int, _DP], ctypes.c_int),
     "le2d_batch_autotune": ([_VP, ctypes.c_int, _IP, _IP, _DP], ctypes.c_int),
+    "le2d_batch_view": ([_VP, ctypes.c_int, ctypes.POINTER(le2d_view)], ctypes.c_int),
+    "le2d_diff_norms": ([ctypes.POINTER(le2d_view), ctypes.POINTER(le2d_view), _DP],
+                        ctypes.c_int),
 }
 
 _libs = {}
@@ -108,6 +120,17 @@ def copy_probe(nbytes, iters=20, dtype=np.float64):
     if rc != 0:
         raise LE2DError(f"copy_probe: {rc} {lib.le2d_last_error().decode()}")
     return ms.value
+
+
+def diff_norms(a, ref):
+    """(||a-ref||_2, ||ref||_2, max|a-ref|, max|ref|) of two device views,
+    FP64 on the device."""
+    lib = load(np.float64)
+    out = (ctypes.c_double * 4)()
+    rc = lib.le2d_diff_norms(ctypes.byref(a), ctypes.byref(ref), out)
+    if rc != 0:
+        raise LE2DError(f"diff_norms: {rc} {lib.le2d_last_error().decode()}")
+    return tuple(out)
 
 
 class Task:
@@ -218,6 +241,27 @@ class Task:
     def device_bytes(self):
         return self.lib.le2d_device_bytes(self.ref)
 
+    def set_materials(self, c1_c2_rho, ids=None):
+        """c1_c2_rho: (count, 3) floats; ids: (ny, nx) uint8 or None."""
+        tab = np.ascontiguousarray(c1_c2_rho, dtype=np.float64).reshape(-1, 3)
+        self._tab = tab
+        self._ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint8)
+        if self._ids is not None:
+            assert self._ids.shape == (self.ny, self.nx)
+        self._check(self.lib.le2d_set_material_plane(
+            self.ref, len(tab), tab.ctypes.data,
+            None if self._ids is None else self._ids.ctypes.data), "set_material_plane")
+
+    def device_view(self):
+        v = le2d_view()
+        self._check(self.lib.le2d_task_view(self.ref, ctypes.byref(v)), "le2d_task_view")
+        return v
+
+    def energy(self):
+        e = ctypes.c_double()
+        self._check(self.lib.le2d_energy(self.ref, ctypes.byref(e)), "le2d_energy")
+        return e.value
+
     def close(self):
         if self._open:
             self.lib.le_free_task(self.ref)
@@ -301,6 +345,11 @@ class Batch:
         self._check(self.lib.le2d_batch_time(self.h, steps, int(plain), ctypes.byref(ms)),
                     "le2d_batch_time")
         return ms.value
+
+    def device_view(self, i):
+        v = le2d_view()
+        self._check(self.lib.le2d_batch_view(self.h, i, ctypes.byref(v)), "le2d_batch_view")
+        return v
 
     def autotune(self, trials=3):
         fb, seg, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()

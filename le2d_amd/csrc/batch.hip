@@ -532,4 +532,18 @@ int le2d_batch_autotune(le2d_batch *b, int trials, int *best_fb, int *best_seg, 
   return le2d::OK;
 }
 
+int le2d_batch_view(le2d_batch *b, int i, le2d_view *v) {
+  if (!b || !b->impl.committed || i < 0 || i >= (int)b->impl.inst.size())
+    return le2d::set_error(le2d::ERR_ARG, "batch: view instance %d", i);
+  const BatchImpl &B = b->impl;
+  const le2d::Inst &in = B.inst[i];
+  v->base = B.arena + in.off[B.parity];
+  v->elem = B.elem;
+  v->nx = in.nx;
+  v->ny = in.ny;
+  v->pitch = in.pitch;
+  v->plane = in.plane;
+  return le2d::OK;
+}
+
 }  // extern "C"

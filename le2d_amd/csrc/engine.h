@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace le2d {
 
@@ -82,5 +83,41 @@ int step_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy, int edge
 int copy_probe(size_t bytes, int iters, double *ms);
 
 Tiles default_tiles(int elem, int nx, int ny);
+
+// Heterogeneous media.  A material plane assigns each node one of `count`
+// materials (c1, c2, rho); see media.hip for the scheme.
+struct MatTile {
+  int strip, j0, j1, mat;  // mat >= 0: the tile's one material; -1: mixed
+};
+
+struct MatPlane {
+  bool active = false;
+  int count = 0;
+  std::vector<double> c1c2rho;     // 3 per material
+  std::vector<unsigned char> ids;  // host copy, nx*ny
+  unsigned char *d_ids = nullptr;  // device copy, rows of idp bytes
+  long idp = 0;
+  void *d_cx = nullptr, *d_cy = nullptr;  // SweepCoef<T> per material
+  double *d_etab = nullptr;               // (rho, lambda, mu) per material
+  MatTile *d_homog = nullptr, *d_mixed = nullptr;
+  int n_homog = 0, n_mixed = 0;
+  int built_fb = -1, built_seg = -1, built_edge = -1;
+};
+
+int mat_set(DevGrid *g, MatPlane *m, int count, const double *c1c2rho, const unsigned char *ids,
+            double dt, double hx, double hy);
+void mat_release(MatPlane *m);
+template <typename T>
+int step_xy_mat(DevGrid *g, MatPlane *m, int edge);
+
+// FP64 norms of a - b over all fields: out = {||a-b||_2, ||b||_2,
+// max|a-b|, max|b|}.  a and b are device states (planes of pitch p*, plane
+// stride q*, element size e*) of the same nx x ny grid.
+int diff_norms(const void *a, int ea, long pa, long qa, const void *b, int eb, long pb, long qb,
+               int nx, int ny, hipStream_t stream, double out[4]);
+
+// Sum over the grid of the elastic energy density in FP64 (per-node material
+// from the plane if active, else the given homogeneous material).
+int energy(DevGrid *g, const MatPlane *m, double rho, double c1, double c2, double *out);
 
 }  // namespace le2d

@@ -29,6 +29,7 @@ using le2d::SweepCoef;
 
 struct TaskState {
   DevGrid g;
+  le2d::MatPlane mat;       // per-node materials, when set
   bool dev_valid = false;   // device buffers hold a state
   bool host_dirty = true;   // mirror newer than device
   bool dev_newer = false;   // device newer than mirror
@@ -128,9 +129,14 @@ int run(le_task *t, int what, int edge, int steps) {
   int rc = ensure_device(t, s);
   if (rc) return rc;
   const SweepCoef<real> cx = coef(t, t->h.x), cy = coef(t, t->h.y);
+  if (s->mat.active && what != (DO_X | DO_Y))
+    return le2d::set_error(LE2D_ERR_STATE,
+                           "single sweeps are not available with a material plane; use le_step");
   for (int k = 0; k < steps; ++k) {
     if (what == (DO_X | DO_Y)) {
-      rc = le2d::step_xy<real>(&s->g, cx, cy, edge);  // one pass per step
+      // one pass per step
+      rc = s->mat.active ? le2d::step_xy_mat<real>(&s->g, &s->mat, edge)
+                         : le2d::step_xy<real>(&s->g, cx, cy, edge);
       if (rc) return rc;
       continue;
     }
@@ -219,6 +225,7 @@ void le_free_task(le_task *task) {
   }
   if (s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    le2d::mat_release(&s->mat);
     le2d::grid_free(&s->g);
     if (s->stream) (void)hipStreamDestroy(s->stream);
   }
@@ -446,6 +453,53 @@ int le2d_time_steps(le_task *task, int steps, double *ms_step) {
   if (rc) return rc;
   *ms_step = tot / steps;
   return LE2D_OK;
+}
+
+int le2d_set_material_plane(le_task *task, int count, const double *c1_c2_rho,
+                            const unsigned char *ids) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  if (count == 0) {
+    le2d::mat_release(&s->mat);
+    s->mat.active = false;
+    return LE2D_OK;
+  }
+  return le2d::mat_set(&s->g, &s->mat, count, c1_c2_rho, ids, (double)task->dt,
+                       (double)task->h.x, (double)task->h.y);
+}
+
+int le2d_energy(le_task *task, double *e) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  double sum = 0.0;
+  rc = le2d::energy(&s->g, &s->mat, (double)task->mat.rho, (double)task->mat.c1,
+                    (double)task->mat.c2, &sum);
+  if (rc) return rc;
+  *e = sum * (double)task->h.x * (double)task->h.y;
+  return LE2D_OK;
+}
+
+int le2d_task_view(le_task *task, le2d_view *v) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  v->base = s->g.buf[s->g.cur];
+  v->elem = s->g.elem;
+  v->nx = s->g.nx;
+  v->ny = s->g.ny;
+  v->pitch = s->g.pitch;
+  v->plane = s->g.plane;
+  return LE2D_OK;
+}
+
+int le2d_diff_norms(const le2d_view *a, const le2d_view *ref, double out[4]) {
+  if (!a || !ref || a->nx != ref->nx || a->ny != ref->ny)
+    return le2d::set_error(LE2D_ERR_ARG, "diff_norms: views of different grids");
+  (void)hipDeviceSynchronize();  // the views' owners may have work queued on their streams
+  return le2d::diff_norms(a->base, a->elem, a->pitch, a->plane, ref->base, ref->elem, ref->pitch,
+                          ref->plane, a->nx, a->ny, nullptr, out);
 }
 
 int le2d_copy_probe(size_t bytes, int iters, double *ms) {

@@ -287,4 +287,77 @@ __device__ __forceinline__ void march_tile(const T *__restrict__ src, T *__restr
 #undef ROW
 }
 
+// ---------------------------------------------------------------------------
+// Heterogeneous media: node-local march for tiles that straddle a material
+// interface.
+//
+// The reference has one material per task (simple/le_core.h:168-186) and no
+// interface rule.  Here every node is updated entirely in its own material:
+// the five stencil nodes of a sweep are projected on that node's
+// eigenvectors, advected with that node's Courant numbers and projected back
+// with that node's constants.  Both faces of a node are therefore evaluated
+// by the node itself (no sharing across an interface); with one material the
+// operands and operations are exactly those of march_tile, so the bits agree.
+// `ids` holds one material index per node (pitched rows of `idp` bytes);
+// cxt/cyt are the per-material constants of the x- and y-sweep.
+template <typename T, int EDGE, int FB>
+__device__ __forceinline__ void march_tile_mixed(const T *__restrict__ src, T *__restrict__ dst,
+                                                 const unsigned char *__restrict__ ids, long idp,
+                                                 int nx, int ny, long pitch, long plane, int strip,
+                                                 int j0, int j1,
+                                                 const SweepCoef<T> *__restrict__ cxt,
+                                                 const SweepCoef<T> *__restrict__ cyt,
+                                                 Node<T> *sn) {
+  const int t = threadIdx.x;
+  const int x = strip * (FB - 4) - 2 + t;
+  const int xc = min(max(x, 0), nx - 1);
+  const unsigned boff = (unsigned)xc * (unsigned)sizeof(T);
+  const unsigned woff = (unsigned)max(x, 0) * (unsigned)sizeof(T);
+  const bool inner = t >= 2 && t <= FB - 3;
+  const bool writer = inner && x < nx;
+
+  // x-sweep of the row of position p; returns this lane's post-x node.
+  auto xrow = [&](int p) -> Node<T> {
+    const int row = ypos<EDGE>(p, ny);
+    Node<T> u = load_row_node(src + (long)row * pitch, plane, boff);
+    const int m = ids[(long)row * idp + xc];
+    sn[t] = u;
+    __syncthreads();
+    if (inner) {
+      const SweepCoef<T> c = cxt[m];
+      W4<T> w[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        const Node<T> &v = sn[t - 2 + q];
+        w[q] = invariants(v.vx, v.sxx, v.vy, v.sxy, c);
+      }
+      const W4<T> fl = faces(w[0], w[1], w[2], w[3], c);
+      const W4<T> fr = faces(w[1], w[2], w[3], w[4], c);
+      apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w[2], fl, fr, c);
+    }
+    __syncthreads();
+    return u;
+  };
+
+  // y-march over positions; P[q] holds the post-x node of position j-2+q
+  Node<T> P[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) P[q] = xrow(j0 - 2 + q);
+  for (int j = j0;;) {
+    const SweepCoef<T> c = cyt[ids[(long)j * idp + xc]];
+    W4<T> w[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) w[q] = inv_y(P[q], c);
+    const W4<T> fl = faces(w[0], w[1], w[2], w[3], c);
+    const W4<T> fr = faces(w[1], w[2], w[3], w[4], c);
+    Node<T> u = P[2];
+    apply(u.vy, u.vx, u.syy, u.sxx, u.sxy, w[2], fl, fr, c);
+    if (writer) store_row_node(dst + (long)j * pitch, plane, woff, u);
+    if (++j >= j1) break;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) P[q] = P[q + 1];
+    P[4] = xrow(j + 2);
+  }
+}
+
 }  // namespace le2d

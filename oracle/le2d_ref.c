@@ -307,5 +307,73 @@ int orc_save_vtk(const real_t *g, int nx, int ny, int soa, real_t hx, real_t hy,
 	return 0;
 }
 
+/*
+ * Heterogeneous media (no counterpart in the reference, which has one
+ * material per task): every node is updated in its own material.  For the
+ * sweep along `ax`, node i of a line with material m = ids[i] projects the
+ * five stencil nodes i-2 .. i+2 (edge handling as orc_line) on m's
+ * invariants, applies the reference's tvd2 with m's Courant numbers
+ * dt*c/h and projects back with m's constants.  With one material this is
+ * orc_line operation for operation.  mats[k] holds material k's constants
+ * (orc_material_init).
+ */
+static void orc_line_mat(const orc_material *mats, const unsigned char *ids, long is, real_t *line,
+                         int n, long ns, long fs, int ax, real_t dt, real_t h, int edge,
+                         real_t *save)
+{
+	int i, q, p;
+	/* pre-sweep copy of the line: save[5*p + f] */
+	for (p = 0; p < n; p++)
+		for (q = 0; q < 5; q++) save[5 * p + q] = line[(long)p * ns + q * fs];
+	for (i = 0; i < n; i++) {
+		const orc_material *m = &mats[ids[(long)i * is]];
+		const real_t k1 = dt * m->c1 / h, k2 = dt * m->c2 / h;
+		real_t w[5][4], d[4];
+		for (q = 0; q < 5; q++) {
+			int pos = i - 2 + q, src;
+			if (pos < 0) src = 0;
+			else if (pos < n) src = pos;
+			else src = (edge == ORC_EDGE_CLAMP) ? n - 1 : (pos - 5 > 0 ? pos - 5 : 0);
+			orc_invariants(m, save + 5 * src, 1, ax, w[q]);
+		}
+		d[0] = orc_tvd2(k1, w[0][0], w[1][0], w[2][0], w[3][0]) - w[2][0];
+		d[1] = orc_tvd2(k1, w[4][1], w[3][1], w[2][1], w[1][1]) - w[2][1];
+		d[2] = orc_tvd2(k2, w[0][2], w[1][2], w[2][2], w[3][2]) - w[2][2];
+		d[3] = orc_tvd2(k2, w[4][3], w[3][3], w[2][3], w[1][3]) - w[2][3];
+		orc_apply(m, line + (long)i * ns, fs, ax, d);
+	}
+}
+
+/* One step (x then y) of the heterogeneous scheme on an AOS or SOA grid;
+ * ids[i + j*nx] < count; c1c2rho holds count (c1, c2, rho) triples. */
+int orc_step_mat(real_t *g, int nx, int ny, int soa, real_t dt, real_t hx, real_t hy, int count,
+                 const double *c1c2rho, const unsigned char *ids, int edge)
+{
+	long ns, fs;
+	int i, j, k;
+	orc_material *mats;
+	if (nx < 3 || ny < 3 || count < 1) return 1;
+	mats = malloc(sizeof(orc_material) * (size_t)count);
+	for (k = 0; k < count; k++)
+		orc_material_init(c1c2rho[3 * k], c1c2rho[3 * k + 1], c1c2rho[3 * k + 2], &mats[k]);
+	orc_strides(nx, ny, soa, &ns, &fs);
+#pragma omp parallel for schedule(static)
+	for (j = 0; j < ny; j++) {
+		real_t *save = malloc(sizeof(real_t) * 5 * (size_t)nx);
+		orc_line_mat(mats, ids + (long)j * nx, 1, g + (long)j * nx * ns, nx, ns, fs, 0, dt, hx,
+		             ORC_EDGE_CLAMP, save);
+		free(save);
+	}
+#pragma omp parallel for schedule(static)
+	for (i = 0; i < nx; i++) {
+		real_t *save = malloc(sizeof(real_t) * 5 * (size_t)ny);
+		orc_line_mat(mats, ids + i, nx, g + (long)i * ns, ny, (long)nx * ns, fs, 1, dt, hy, edge,
+		             save);
+		free(save);
+	}
+	free(mats);
+	return 0;
+}
+
 /* Size of real_t, so a loader can check which build it opened. */
 int orc_real_size(void) { return (int)sizeof(real_t); }

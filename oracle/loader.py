@@ -52,6 +52,8 @@ class Oracle:
         L.orc_set_ball.restype = ctypes.c_long
         L.orc_save_vtk.argtypes = [P(R), ctypes.c_int, ctypes.c_int, ctypes.c_int, R, R,
                                    ctypes.c_char_p]
+        L.orc_step_mat.argtypes = grid_args + [R, R, R, ctypes.c_int, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_int]
         assert L.orc_real_size() == self.dtype.itemsize
 
     # --- scalar pieces -----------------------------------------------------
@@ -94,6 +96,31 @@ class Oracle:
         rc = self.lib.orc_run(_ptr(g, self.real), nx, ny, soa, dt, h[0], h[1],
                               ctypes.byref(self._mat(mat)), steps)
         assert rc == 0
+
+    def run_mat(self, g, dt, h, c1c2rho, ids, steps, edge=EDGE_CLAMP):
+        """Heterogeneous scheme: ids (ny, nx) uint8 material indices into the
+        (count, 3) table of (c1, c2, rho)."""
+        nx, ny, soa = self._grid(g)
+        tab = np.ascontiguousarray(c1c2rho, dtype=np.float64).reshape(-1, 3)
+        ids = np.ascontiguousarray(ids, dtype=np.uint8)
+        assert ids.shape == (ny, nx)
+        for _ in range(steps):
+            rc = self.lib.orc_step_mat(_ptr(g, self.real), nx, ny, soa, dt, h[0], h[1], len(tab),
+                                       tab.ctypes.data, ids.ctypes.data, edge)
+            assert rc == 0
+
+    def energy(self, g, h, mat):
+        """FP64 elastic energy of an AOS grid (homogeneous material)."""
+        m = self._mat(mat)
+        v = g.reshape(-1, 5).astype(np.float64)
+        rho, c1, c2 = float(m.rho), float(m.c1), float(m.c2)
+        mu = rho * c2 * c2
+        lam = rho * c1 * c1 - 2.0 * mu
+        tr = v[:, 2] + v[:, 4]
+        dv = v[:, 2] - v[:, 4]
+        dens = (0.5 * rho * (v[:, 0] ** 2 + v[:, 1] ** 2) + tr * tr / (8.0 * (lam + mu))
+                + (0.5 * dv * dv + 2.0 * v[:, 3] ** 2) / (4.0 * mu))
+        return float(dens.sum()) * h[0] * h[1]
 
     def set_ball(self, g, h, c, r, s):
         nx, ny, soa = self._grid(g)
