@@ -1,16 +1,26 @@
 """Throughput of the split-step elasticity sweeps on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--dtype f64|f32] [--n 8192]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 8192]
+                    [--no-cpu] [--no-batch] [--no-tier] [--no-fp32]
 
-Workload (the plain GPU sweep at scale): an n x n grid (default 8192^2),
-material c1 = 2, c2 = 1, rho = 1, h = 1, dt = 0.5*h/c1 = 0.25, all fields
-zero except a disk of radius 64 cells (sxx = syy = 1) at a seeded position in
-[64, n-64)^2.  A step is one x-sweep plus one y-sweep over the whole grid,
-driven through the le_core.h C ABI (le_step_n) with the fields resident in
-HBM.  W untimed steps, then exactly K steps timed between two completion
-waits.  Prints one JSON line.
+Headline workload (`value`): the plain GPU sweep at scale.  An n x n grid
+(default 8192^2), FP64, material c1 = 2, c2 = 1, rho = 1, h = 1,
+dt = 0.5*h/c1 = 0.25, all fields zero except a disk of radius 64 cells
+(sxx = syy = 1) at a seeded position in [64, n-64)^2.  A step is one x-sweep
+plus one y-sweep over the whole grid, driven through the le_core.h C ABI
+(le_step_n) with the fields resident in HBM.  W untimed steps, then exactly K
+steps timed between two completion waits.
+
+Further legs (each in its own key of the one JSON line printed):
+  fp32     the same workload in FP32, its error against the FP64 run
+  batched  256 instances of mixed sizes and materials: one persistent
+           launch per step vs one launch per instance (FP32 and FP64)
+  tier     16384^2, two materials split at the midline, 2000 steps, FP32 vs
+           FP64 field error and energy drift
+  cpu_baseline  the reference's own simple/ path on the host cores
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -25,12 +35,66 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 METRIC = "Gnode-updates/s & % HBM peak, batched vs plain at stated error; p50/p99 latency"
 
 
-def make_task(le2d_amd, n, dtype, seed=1):
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def disk_centre(n, seed=1):
     rng = np.random.default_rng(seed)
-    cx, cy = (float(v) for v in rng.integers(64, n - 64, size=2))
+    return tuple(float(v) for v in rng.integers(64, n - 64, size=2))
+
+
+def plain_task(le2d_amd, n, dtype):
     t = le2d_amd.Task(n, n, 0.25, (1.0, 1.0), (2.0, 1.0, 1.0), dtype=dtype)
-    t.set_ball((cx, cy), 64.0, 1.0)
-    return t, (cx, cy)
+    t.set_ball(disk_centre(n), 64.0, 1.0)
+    return t
+
+
+def run_plain(le2d_amd, n, dtype, steps, warmup, profile_steps):
+    """Times `steps` steps; then times the fused kernel and the two sweep
+    kernels per launch with HIP events.  Every run of this function advances
+    the task by warmup + steps + 2*profile_steps steps."""
+    t = plain_task(le2d_amd, n, dtype)
+    t.step_n(max(1, warmup))
+    t.wait()
+    t0 = time.perf_counter()
+    t.step_n(steps)
+    t.wait()
+    el = time.perf_counter() - t0
+    ms_step = t.time_steps(profile_steps)
+    ms_x, ms_y = t.time_sweeps(profile_steps)
+    return t, el, ms_step, ms_x, ms_y
+
+
+def measured_traffic(n, dtype):
+    """HBM bytes per k_step_xy launch from the committed rocprofv3 counter
+    summary for this workload (profiles/traffic_k_step_xy_<n>_<dtype>.json),
+    or None when no such measurement exists."""
+    p = os.path.join(ROOT, "profiles", f"traffic_k_step_xy_{n}_{dtype}.json")
+    if not os.path.exists(p):
+        return None
+    return json.load(open(p)).get("hbm_bytes_per_launch")
+
+
+def roofline(n, es, ms_step, ms_x, ms_y, dtype_name):
+    nodes = n * n
+    # Algorithmic bytes per node-update: 2 sweeps x (5 reads + 5 writes) x
+    # sizeof(real); one k_step_xy launch advances every node by one step.
+    alg = nodes * 20 * es
+    achieved = alg / (ms_step * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": "k_step_xy", "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": measured_traffic(n, dtype_name),
+            "algorithmic_bytes_per_launch": alg,
+            "ms_per_launch": {"k_step_xy": round(ms_step, 4), "k_sweep_x": round(ms_x, 4),
+                              "k_sweep_y": round(ms_y, 4)},
+            "two_kernel_GBps": {"k_sweep_x": round(nodes * 10 * es / ms_x / 1e6, 1),
+                                "k_sweep_y": round(nodes * 10 * es / ms_y / 1e6, 1)}}
+
+
+def rel_errors(norms):
+    e2, r2, einf, rinf = norms
+    return {"l2_rel": e2 / r2 if r2 > 0 else None, "linf_rel": einf / rinf if rinf > 0 else None}
 
 
 def cpu_baseline(n, budget_s=12.0):
@@ -38,10 +102,8 @@ def cpu_baseline(n, budget_s=12.0):
     sources) if present, else the oracle port; FP64, same grid and disk, a
     bounded number of steps on all host cores."""
     from oracle import Oracle, Reference
-    import ctypes
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    rng = np.random.default_rng(1)
-    cx, cy = (float(v) for v in rng.integers(64, n - 64, size=2))
+    cx, cy = disk_centre(n)
     if Reference.available(np.float64):
         ref = Reference(np.float64)
         t = ref.task(n, n, 0.25, (1.0, 1.0), (2.0, 1.0, 1.0))
@@ -71,20 +133,14 @@ def cpu_baseline(n, budget_s=12.0):
                       f"{el:.1f} s wall, OMP_NUM_THREADS={cores}"}
 
 
-def measured_traffic(n, dtype):
-    """HBM bytes per k_step_xy launch from the committed rocprofv3 counter
-    summary for this workload (profiles/traffic_k_step_xy_<n>_<dtype>.json),
-    or None when no such measurement exists."""
-    p = os.path.join(ROOT, "profiles", f"traffic_k_step_xy_{n}_{dtype}.json")
-    if not os.path.exists(p):
-        return None
-    return json.load(open(p)).get("hbm_bytes_per_launch")
+# ---------------------------------------------------------------------------
+# Mixed-size batch
 
 
 def mixed_instances(count=256, seed=3):
-    """Mixed-size workload: sides log-uniform in [128, 2048] (independent),
-    c1 ~ U[1,4], c2 = c1/U[1.5,2.2], rho ~ U[1,3], dt = 0.5*h/c1 with h = 1,
-    disk radius U[4,32] inside the grid, sxx = syy = 1."""
+    """Sides log-uniform in [128, 2048] (independent), c1 ~ U[1,4],
+    c2 = c1/U[1.5,2.2], rho ~ U[1,3], dt = 0.5*h/c1 with h = 1, disk radius
+    U[4,32] inside the grid, sxx = syy = 1."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
@@ -100,7 +156,7 @@ def mixed_instances(count=256, seed=3):
 
 def run_batch(le2d_amd, dtype, insts, steps, plain_steps):
     """Batched launch (autotuned, graph-replayed) vs one plain launch per
-    instance per step, same instances.  Returns (result dict, batch)."""
+    instance per step, same instances."""
     b = le2d_amd.Batch(dtype)
     for p in insts:
         b.add(p["nx"], p["ny"], p["dt"], (1.0, 1.0), (p["c1"], p["c2"], p["rho"]))
@@ -124,37 +180,70 @@ def run_batch(le2d_amd, dtype, insts, steps, plain_steps):
     return res, b
 
 
-def field_errors(a, ref):
-    """Relative L2 and Linf of `a` against `ref` over all fields (FP64 sums)."""
-    d = a.astype(np.float64) - ref
-    return (float(np.sqrt(np.sum(d * d))), float(np.sqrt(np.sum(ref * ref))),
-            float(np.max(np.abs(d))), float(np.max(np.abs(ref))))
-
-
 def batch_report(le2d_amd, steps=500, plain_steps=50, count=256):
     insts = mixed_instances(count)
     out = {"workload": f"{count} instances, sides log-uniform [128, 2048], per-instance material "
-                       f"and disk, {steps} steps"}
+                       f"and disk, {steps} steps timed batched, {plain_steps} per-instance"}
     r64, b64 = run_batch(le2d_amd, np.float64, insts, steps, plain_steps)
     r32, b32 = run_batch(le2d_amd, np.float32, insts, steps, plain_steps)
-    # both batches advanced steps + plain_steps (+1 warm-up inside time()) from the disk
-    e2n = e2d = einf = rinf = 0.0
-    worst = 0.0
+    # both advanced the same number of steps from the same disks
+    e2 = r2 = 0.0
+    einf = rinf = worst = 0.0
     for i in range(len(insts)):
-        ref = b64.read(i)
-        a = b32.read(i)
-        n2, d2, ni, di = field_errors(a, ref)
-        e2n += n2 * n2
-        e2d += d2 * d2
-        einf, rinf = max(einf, ni), max(rinf, di)
-        worst = max(worst, n2 / d2 if d2 > 0 else 0.0)
+        n = le2d_amd.diff_norms(b32.device_view(i), b64.device_view(i))
+        e2 += n[0] ** 2
+        r2 += n[1] ** 2
+        einf, rinf = max(einf, n[2]), max(rinf, n[3])
+        worst = max(worst, n[0] / n[1] if n[1] > 0 else 0.0)
     b64.close()
     b32.close()
-    r32["l2_rel_vs_fp64"] = float(np.sqrt(e2n / e2d))
+    r32["l2_rel_vs_fp64"] = float(np.sqrt(e2 / r2))
     r32["l2_rel_vs_fp64_worst_instance"] = worst
     r32["linf_rel_vs_fp64"] = einf / rinf
     out["fp32"] = r32
     out["fp64"] = r64
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Precision tier with two materials
+
+
+def tier_report(le2d_amd, n=16384, steps=2000, seed=7):
+    rng = np.random.default_rng(seed)
+    mats = []
+    for _ in range(2):
+        c1 = float(rng.uniform(1.5, 3.0))
+        mats.append((c1, c1 / float(rng.uniform(1.6, 2.0)), float(rng.uniform(1.0, 3.0))))
+    dt = 0.5 / max(m[0] for m in mats)
+    ids = np.zeros((n, n), np.uint8)
+    ids[:, n // 2:] = 1
+    cx, cy = disk_centre(n, seed)
+    out = {"workload": f"{n}x{n}, two materials (c1, c2, rho) = "
+                       f"{tuple(round(v, 4) for v in mats[0])} | "
+                       f"{tuple(round(v, 4) for v in mats[1])} split at x = {n // 2}, "
+                       f"dt = {dt:.4f}, disk r = 64 at ({cx:.0f}, {cy:.0f}), {steps} steps"}
+    tasks = {}
+    for name, dtype in (("fp64", np.float64), ("fp32", np.float32)):
+        log(f"tier {name}")
+        t = le2d_amd.Task(n, n, dt, (1.0, 1.0), mats[0], dtype=dtype)
+        t.set_ball((cx, cy), 64.0, 1.0)
+        t.set_materials(mats, ids)
+        e0 = t.energy()
+        t.wait()
+        t0 = time.perf_counter()
+        t.step_n(steps)
+        t.wait()
+        el = time.perf_counter() - t0
+        e1 = t.energy()
+        out[name] = {"Gnode_per_s": round(n * n * steps / el / 1e9, 3), "seconds": round(el, 3),
+                     "energy_initial": e0, "energy_final": e1,
+                     "energy_rel_drift": (e1 - e0) / e0}
+        tasks[name] = t
+    norms = le2d_amd.diff_norms(tasks["fp32"].device_view(), tasks["fp64"].device_view())
+    out["fp32"].update({"vs_fp64_" + k: v for k, v in rel_errors(norms).items()})
+    for t in tasks.values():
+        t.close()
     return out
 
 
@@ -163,13 +252,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--dtype", choices=["f64", "f32"], default="f64",
+                    help="type of the headline run (the other type runs as a second leg)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--profile-steps", type=int, default=20,
-                    help="steps timed per sweep with HIP events for the roofline")
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch leg")
+    ap.add_argument("--no-tier", action="store_true", help="skip the two-material FP32 tier leg")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the second-type run")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--tier-steps", type=int, default=2000)
+    ap.add_argument("--profile-steps", type=int, default=20,
+                    help="launches timed per kernel with HIP events for the roofline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -177,70 +270,63 @@ def main():
         sys.exit(2)
 
     import le2d_amd
-    dtype = np.float64 if args.dtype == "f64" else np.float32
-    es = np.dtype(dtype).itemsize
     n = args.n
-    print(f"[bench] {n}x{n} {args.dtype}: building task", file=sys.stderr, flush=True)
-    t, centre = make_task(le2d_amd, n, dtype)
-    t.step_n(max(1, args.warmup))  # uploads, then warms up
-    t.wait()
-    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
-    t.wait()
-    t0 = time.perf_counter()
-    t.step_n(args.steps)
-    t.wait()
-    el = time.perf_counter() - t0
-    nodes = n * n
-    value = nodes * args.steps / el / 1e9
-
-    # Per-launch durations with HIP events on the task's stream.  le_step runs
-    # one kernel per step (k_step_xy); the separate sweep kernels (le_step_x,
-    # le_step_y) are timed too for reference.
-    ms_step = t.time_steps(args.profile_steps)
-    ms_x, ms_y = t.time_sweeps(args.profile_steps)
-    # Algorithmic bytes per node-update: 2 sweeps x (5 reads + 5 writes) x
-    # sizeof(real); one k_step_xy launch advances every node by one step.
-    alg_bytes = nodes * 20 * es
-    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
-    traffic = measured_traffic(n, args.dtype)
-    roofline = {"bound": "hbm", "kernel": "k_step_xy", "achieved": round(achieved, 1),
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                "traffic": traffic,
-                "algorithmic_bytes_per_launch": alg_bytes,
-                "ms_per_launch": {"k_step_xy": round(ms_step, 4), "k_sweep_x": round(ms_x, 4),
-                                  "k_sweep_y": round(ms_y, 4)},
-                "two_kernel_GBps": {"k_sweep_x": round(nodes * 10 * es / ms_x / 1e6, 1),
-                                    "k_sweep_y": round(nodes * 10 * es / ms_y / 1e6, 1)}}
+    main_dt = np.float64 if args.dtype == "f64" else np.float32
+    es = np.dtype(main_dt).itemsize
+    log(f"{n}x{n} {args.dtype}: {args.warmup} warm-up + {args.steps} timed steps")
+    t, el, ms_step, ms_x, ms_y = run_plain(le2d_amd, n, main_dt, args.steps, args.warmup,
+                                           args.profile_steps)
+    value = n * n * args.steps / el / 1e9
+    roof = roofline(n, es, ms_step, ms_x, ms_y, args.dtype)
     tiles = t.get_tiles()
+    other = None
+    if not args.no_fp32:
+        o_dt = np.float32 if main_dt == np.float64 else np.float64
+        o_name = "f32" if o_dt == np.float32 else "f64"
+        log(f"{n}x{n} {o_name}")
+        t2, el2, ms2, mx2, my2 = run_plain(le2d_amd, n, o_dt, args.steps, args.warmup,
+                                           args.profile_steps)
+        lo, hi = (t2, t) if o_dt == np.float32 else (t, t2)
+        errs = rel_errors(le2d_amd.diff_norms(lo.device_view(), hi.device_view()))
+        other = {"dtype": o_name, "value": round(n * n * args.steps / el2 / 1e9, 3),
+                 "ms_per_step": round(el2 / args.steps * 1e3, 4),
+                 "roofline": roofline(n, np.dtype(o_dt).itemsize, ms2, mx2, my2, o_name),
+                 "fp32_vs_fp64_after_steps": args.warmup + args.steps + 2 * args.profile_steps,
+                 "fp32_vs_fp64": errs}
+        t2.close()
     t.close()
-    copy_ms = le2d_amd.copy_probe(nodes * 5 * es, 10)
-    copy_gbs = 2 * nodes * 5 * es / copy_ms / 1e6
+    copy_bytes = n * n * 5 * es
+    copy_gbs = 2 * copy_bytes / le2d_amd.copy_probe(copy_bytes, 10) / 1e6
 
     batched = None
     if not args.no_batch:
-        print("[bench] mixed-size batch", file=sys.stderr, flush=True)
+        log("mixed-size batch")
         batched = batch_report(le2d_amd, steps=args.batch_steps)
-
+    tier = None
+    if not args.no_tier:
+        tier = tier_report(le2d_amd, steps=args.tier_steps)
     cpu = None
     if not args.no_cpu:
-        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        log("CPU baseline")
         cpu = cpu_baseline(n)
 
+    cx, cy = disk_centre(n)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gnode-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(el / args.steps * 1e3, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if dtype == np.float64 else "f32", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": args.dtype, "data": "synthetic",
         "config": {"workload": f"plain split-step sweep, {n}x{n} grid, disk r=64 at "
-                               f"({centre[0]:.0f},{centre[1]:.0f}), c1=2 c2=1 rho=1, dt=0.25",
-                   "nx": n, "ny": n, "tiles": {"x_rows": tiles[0], "y_seg": tiles[1],
-                                               "y_block": tiles[2], "xy_seg": tiles[3],
-                                               "xy_block": tiles[4]}},
+                               f"({cx:.0f},{cy:.0f}), c1=2 c2=1 rho=1, dt=0.25",
+                   "nx": n, "ny": n,
+                   "tiles": dict(zip(("x_rows", "y_seg", "y_block", "xy_seg", "xy_block"),
+                                     tiles))},
         "hbm_copy_GBps": round(copy_gbs, 1),
-        "roofline": roofline,
+        "roofline": roof,
         "cpu_baseline": cpu,
+        ("fp32" if main_dt == np.float64 else "fp64"): other,
         "batched": batched,
+        "tier": tier,
     }
     print(json.dumps(out))
 
