@@ -17,6 +17,9 @@ Further legs (each in its own key of the one JSON line printed):
            launch per step vs one launch per instance (FP32 and FP64)
   tier     16384^2, two materials split at the midline, 2000 steps, FP32 vs
            FP64 field error and energy drift
+  serving  Poisson request stream through the serving loop (continuous
+           batching under an HBM budget) vs the serial plain path: throughput
+           and p50/p99 completion latency
   cpu_baseline  the reference's own simple/ path on the host cores
 """
 import argparse
@@ -247,6 +250,94 @@ def tier_report(le2d_amd, n=16384, steps=2000, seed=7):
     return out
 
 
+# ---------------------------------------------------------------------------
+# Serving loop
+
+
+def serve_requests(window_s, rate=200.0, seed=11):
+    """Poisson arrivals at `rate` per second for `window_s` seconds; sides
+    log-uniform in [256, 4096] (independent), steps U[200, 2000], 70% FP32 /
+    30% FP64, per-request material (c1 ~ U[1,4], c2 = c1/U[1.5,2.2],
+    rho ~ U[1,3], dt = 0.5/c1) and disk (r ~ U[4,32])."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > window_s:
+            return out
+        nx, ny = (int(round(np.exp(rng.uniform(np.log(256), np.log(4096))))) for _ in range(2))
+        c1 = float(rng.uniform(1.0, 4.0))
+        r = float(rng.uniform(4.0, 32.0))
+        out.append(dict(at=t, nx=nx, ny=ny, steps=int(rng.integers(200, 2001)),
+                        dtype=np.float32 if rng.random() < 0.7 else np.float64, dt=0.5 / c1,
+                        mat=(c1, c1 / float(rng.uniform(1.5, 2.2)), float(rng.uniform(1, 3))),
+                        disk=(float(rng.uniform(r, nx - r)), float(rng.uniform(r, ny - r)), r,
+                              1.0)))
+
+
+def pct(v, q):
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def serve_report(le2d_amd, window_s=1.0, budget_gb=200.0, slots=1024):
+    reqs = serve_requests(window_s)
+    work = sum(q["nx"] * q["ny"] * q["steps"] for q in reqs)
+    out = {"workload": f"Poisson 200 req/s for {window_s} s ({len(reqs)} requests, "
+                       f"{work / 1e9:.1f} Gnode-updates), sides log-uniform [256, 4096], steps "
+                       f"U[200, 2000], 70% FP32 / 30% FP64, HBM budget {budget_gb:.0f} GB"}
+    budget = int(budget_gb * (1 << 30))
+
+    def submit_all(s, realtime):
+        ids = []
+        t0 = time.perf_counter()
+        for q in reqs:
+            if realtime:
+                dt = q["at"] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+            ids.append(s.submit(q["nx"], q["ny"], q["steps"], q["dtype"], q["dt"], (1.0, 1.0),
+                                q["mat"], q["disk"]))
+        return ids
+
+    # continuous batching, arrivals in real time
+    log(f"serving {len(reqs)} requests")
+    with le2d_amd.Server(budget, slots) as s:
+        ids = submit_all(s, True)
+        s.drain()
+        times = [s.times(i) for i in ids]
+        st = s.stats()
+    t_first = min(t[0] for t in times)
+    span = max(t[2] for t in times) - t_first
+    lat = [t[2] - t[0] for t in times]
+    out["batched"] = {"completed": st["completed"], "throughput_Gnode_per_s":
+                      round(st["node_updates"] / span / 1e9, 3),
+                      "gpu_busy_Gnode_per_s": round(st["node_updates"] / st["busy_seconds"] / 1e9, 3),
+                      "p50_latency_s": round(pct(lat, 50), 3), "p99_latency_s": round(pct(lat, 99), 3),
+                      "makespan_s": round(span, 3), "peak_live": st["peak_live"],
+                      "peak_GB": round(st["peak_bytes"] / 2 ** 30, 2)}
+    # serial plain path: one request at a time, arrival order; the service
+    # times are measured on the GPU, the queueing follows the same arrivals
+    log("serial baseline")
+    with le2d_amd.Server(budget, 1, plain=True) as s:
+        ids = submit_all(s, False)
+        s.drain()
+        # the engine admits request k+1 when k completes: its service time
+        # (initialisation and steps) is the gap between completions
+        done = [s.times(i)[2] for i in ids]
+        svc = [done[0] - s.times(ids[0])[1]] + [b - a for a, b in zip(done, done[1:])]
+    free, lat_s = 0.0, []
+    for q, d in zip(reqs, svc):
+        start = max(q["at"], free)
+        free = start + d
+        lat_s.append(free - q["at"])
+    out["serial"] = {"throughput_Gnode_per_s": round(work / (free - reqs[0]["at"]) / 1e9, 3),
+                     "p50_latency_s": round(pct(lat_s, 50), 3),
+                     "p99_latency_s": round(pct(lat_s, 99), 3), "makespan_s": round(free, 3)}
+    out["throughput_ratio_batched_vs_serial"] = round(
+        out["batched"]["throughput_Gnode_per_s"] / out["serial"]["throughput_Gnode_per_s"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +350,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the mixed-size batch leg")
     ap.add_argument("--no-tier", action="store_true", help="skip the two-material FP32 tier leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the second-type run")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--serve-window", type=float, default=1.0,
+                    help="seconds of Poisson arrivals in the serving leg")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--tier-steps", type=int, default=2000)
     ap.add_argument("--profile-steps", type=int, default=20,
@@ -305,6 +399,9 @@ def main():
     tier = None
     if not args.no_tier:
         tier = tier_report(le2d_amd, steps=args.tier_steps)
+    serving = None
+    if not args.no_serve:
+        serving = serve_report(le2d_amd, window_s=args.serve_window)
     cpu = None
     if not args.no_cpu:
         log("CPU baseline")
@@ -327,6 +424,7 @@ def main():
         ("fp32" if main_dt == np.float64 else "fp64"): other,
         "batched": batched,
         "tier": tier,
+        "serving": serving,
     }
     print(json.dumps(out))
 

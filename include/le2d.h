@@ -166,6 +166,55 @@ int le2d_batch_autotune(le2d_batch *b, int trials, int *best_fb, int *best_seg, 
 /* View of instance `inst`'s current state (see le2d_view). */
 int le2d_batch_view(le2d_batch *b, int inst, le2d_view *v);
 
+/* ---------------------------------------------------------------------------
+ * Serving loop: simulation requests admitted by HBM budget and slots, fields
+ * resident on the device, the steps of all live requests batched into one
+ * launch per precision per step as they arrive (continuous batching).  No
+ * counterpart in the reference.  A request's result does not depend on the
+ * other requests in flight (bit-identical alone and in any mix). */
+typedef struct {
+  int nx, ny;        /* grid size (>= 3) */
+  int real_size;     /* 4: FP32 tier, 8: FP64 */
+  int steps;         /* full steps to run */
+  double dt, hx, hy; /* time step, spacing */
+  double c1, c2, rho;   /* homogeneous material */
+  double cx, cy, r, s;  /* disk source: sxx = syy = s where |x - c| < r */
+  int keep_result;   /* 1: keep the final state until le2d_server_result */
+} le2d_request;
+
+typedef struct {
+  long long completed, queued;
+  int live, peak_live;
+  double node_updates;   /* node-updates executed so far */
+  double busy_seconds;   /* wall time the engine spent in chunks */
+  size_t bytes_in_use, peak_bytes;
+} le2d_server_stats;
+
+typedef struct le2d_server le2d_server;
+
+/* hbm_budget: device bytes for request fields (one arena, reserved now);
+ * max_live: concurrent requests; plain = 1: serial baseline (one slot, the
+ * single-grid kernel). */
+le2d_server *le2d_server_create(size_t hbm_budget, int max_live, int plain);
+/* Launch shape of the batched steps and the longest run of steps between
+ * admission points. */
+int le2d_server_set_shape(le2d_server *srv, int fb, int seg, int max_chunk);
+/* Queue a request; returns its id (>= 0) or a negative error code
+ * (-LE2D_ERR_BUDGET when it can never fit). */
+long long le2d_server_submit(le2d_server *srv, const le2d_request *req);
+/* Block until request `id` has finished (timeout_s < 0: no limit). */
+int le2d_server_wait(le2d_server *srv, long long id, double timeout_s);
+/* Block until every submitted request has finished. */
+int le2d_server_drain(le2d_server *srv, double timeout_s);
+/* Copy a kept result into an AOS host array (nx*ny nodes of five reals of
+ * the request's type) and release its device memory. */
+int le2d_server_result(le2d_server *srv, long long id, void *aos);
+/* Submission, admission and completion times (seconds, steady clock). */
+int le2d_server_times(le2d_server *srv, long long id, double *t_submit, double *t_admit,
+                      double *t_done);
+int le2d_server_get_stats(le2d_server *srv, le2d_server_stats *out);
+void le2d_server_destroy(le2d_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,20 @@ class le2d_view(ctypes.Structure):
                 ("ny", ctypes.c_int), ("pitch", ctypes.c_long), ("plane", ctypes.c_long)]
 
 
+class le2d_request(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("real_size", ctypes.c_int),
+                ("steps", ctypes.c_int)] + [(n, ctypes.c_double) for n in
+                                            ("dt", "hx", "hy", "c1", "c2", "rho", "cx", "cy",
+                                             "r", "s")] + [("keep_result", ctypes.c_int)]
+
+
+class le2d_server_stats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_longlong), ("queued", ctypes.c_longlong),
+                ("live", ctypes.c_int), ("peak_live", ctypes.c_int),
+                ("node_updates", ctypes.c_double), ("busy_seconds", ctypes.c_double),
+                ("bytes_in_use", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t)]
+
+
 # include/le2d.h batched-instance API (no le_task argument).
 _VP = ctypes.c_void_p
 _IP = ctypes.POINTER(ctypes.c_int)
@@ -72,6 +86,15 @@ _BATCH = {
     "le2d_batch_view": ([_VP, ctypes.c_int, ctypes.POINTER(le2d_view)], ctypes.c_int),
     "le2d_diff_norms": ([ctypes.POINTER(le2d_view), ctypes.POINTER(le2d_view), _DP],
                         ctypes.c_int),
+    "le2d_server_create": ([ctypes.c_size_t, ctypes.c_int, ctypes.c_int], _VP),
+    "le2d_server_set_shape": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "le2d_server_submit": ([_VP, ctypes.POINTER(le2d_request)], ctypes.c_longlong),
+    "le2d_server_wait": ([_VP, ctypes.c_longlong, ctypes.c_double], ctypes.c_int),
+    "le2d_server_drain": ([_VP, ctypes.c_double], ctypes.c_int),
+    "le2d_server_result": ([_VP, ctypes.c_longlong, _VP], ctypes.c_int),
+    "le2d_server_times": ([_VP, ctypes.c_longlong, _DP, _DP, _DP], ctypes.c_int),
+    "le2d_server_get_stats": ([_VP, ctypes.POINTER(le2d_server_stats)], ctypes.c_int),
+    "le2d_server_destroy": ([_VP], None),
 }
 
 _libs = {}
@@ -361,6 +384,77 @@ class Batch:
     def close(self):
         if self.h:
             self.lib.le2d_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Server:
+    """Serving loop (include/le2d.h, le2d_server_*): requests admitted by HBM
+    budget and slots, continuous batching of their steps on the device."""
+
+    def __init__(self, budget_bytes, max_live=256, plain=False):
+        self.lib = load(np.float64)
+        self.h = self.lib.le2d_server_create(int(budget_bytes), max_live, int(plain))
+        if not self.h:
+            raise LE2DError("le2d_server_create: " + self.lib.le2d_last_error().decode())
+        self.shapes = {}
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise LE2DError(f"{what}: {rc} {self.lib.le2d_last_error().decode()}")
+
+    def set_shape(self, fb=256, seg=32, max_chunk=16):
+        self._check(self.lib.le2d_server_set_shape(self.h, fb, seg, max_chunk), "set_shape")
+
+    def submit(self, nx, ny, steps, dtype=np.float32, dt=0.25, h=(1.0, 1.0),
+               mat=(2.0, 1.0, 1.0), disk=(None, None, 10.0, 1.0), keep=False):
+        cx, cy, r, sv = disk
+        q = le2d_request(nx, ny, np.dtype(dtype).itemsize, steps, dt, h[0], h[1], mat[0],
+                         mat[1], mat[2], nx / 2 if cx is None else cx,
+                         ny / 2 if cy is None else cy, r, sv, int(keep))
+        i = self.lib.le2d_server_submit(self.h, ctypes.byref(q))
+        if i < 0:
+            self._check(-i, "le2d_server_submit")
+        self.shapes[i] = ((ny, nx), np.dtype(dtype))
+        return i
+
+    def wait(self, i, timeout=-1.0):
+        self._check(self.lib.le2d_server_wait(self.h, i, timeout), "le2d_server_wait")
+
+    def drain(self, timeout=-1.0):
+        self._check(self.lib.le2d_server_drain(self.h, timeout), "le2d_server_drain")
+
+    def result(self, i):
+        shape, dt = self.shapes[i]
+        a = np.empty(shape + (5,), dtype=dt)
+        self._check(self.lib.le2d_server_result(self.h, i, a.ctypes.data), "le2d_server_result")
+        return a
+
+    def times(self, i):
+        t = [ctypes.c_double() for _ in range(3)]
+        self._check(self.lib.le2d_server_times(self.h, i, *[ctypes.byref(x) for x in t]),
+                    "le2d_server_times")
+        return tuple(x.value for x in t)
+
+    def stats(self):
+        st = le2d_server_stats()
+        self._check(self.lib.le2d_server_get_stats(self.h, ctypes.byref(st)), "stats")
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def close(self):
+        if self.h:
+            self.lib.le2d_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

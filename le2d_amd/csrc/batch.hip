@@ -23,60 +23,10 @@
 #include <tuple>
 #include <vector>
 
+#include "batch_dev.h"
 #include "le2d.h"
-#include "sweep_dev.h"
 
 namespace le2d {
-
-template <typename T>
-struct InstDesc {
-  T *buf[2];
-  int nx, ny;
-  long pitch, plane;
-  SweepCoef<T> cx, cy;
-};
-
-struct TileJob {
-  int inst, strip, j0, j1;
-};
-
-// Persistent over the tile table.  Workgroup b runs on XCD b % 8 (dispatch
-// is round-robin); each XCD walks its own contiguous eighth of the table, so
-// concurrently running workgroups of one XCD process neighbouring strips.
-template <typename T, int FB>
-__global__ __launch_bounds__(FB) void k_step_batch(const InstDesc<T> *__restrict__ inst,
-                                                   const TileJob *__restrict__ tiles, int ntiles,
-                                                   int parity) {
-  __shared__ W4<T> sw[FB];
-  __shared__ W4<T> sf[FB];
-  const int per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
-  const int xcd = blockIdx.x & 7, r = blockIdx.x >> 3;
-  const int lo = (int)((long)ntiles * xcd / 8), hi = (int)((long)ntiles * (xcd + 1) / 8);
-  for (int k = lo + r; k < hi; k += per) {
-    const TileJob tj = tiles[k];
-    const InstDesc<T> &d = inst[tj.inst];
-    march_tile<T, EDGE_CLAMP, FB>(d.buf[parity], d.buf[parity ^ 1], d.nx, d.ny, d.pitch, d.plane,
-                                  tj.strip, tj.j0, tj.j1, d.cx, d.cy, sw, sf);
-  }
-}
-
-// Disk initial condition with the host le_set_ball's arithmetic: the node
-// position and offset in T, the distance as the double square root of the
-// T-rounded sum of squares, strict comparison with r (simple/le_core.c:71-94).
-template <typename T>
-__global__ void k_set_ball(T *__restrict__ buf, int nx, int ny, long pitch, long plane, T hx, T hy,
-                           T cx, T cy, T r, T s) {
-  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= (long)nx * ny) return;
-  const int j = (int)(n / nx), i = (int)(n - (long)j * nx);
-  const T px = hx * (T)i, py = hy * (T)j;
-  const T dx = px - cx, dy = py - cy;
-  const T d2 = dx * dx + dy * dy;
-  if (::sqrt((double)d2) < (double)r) {
-    buf[2 * plane + (long)j * pitch + i] = s;
-    buf[4 * plane + (long)j * pitch + i] = s;
-  }
-}
 
 // ---------------------------------------------------------------------------
 
@@ -96,30 +46,9 @@ struct Shape {
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, long>, Shape> g_tune;  // (elem, instances, Mnodes) -> shape
 
-// Material constants and sweep constants in double (the reference's
-// le_init_material and le_step_x/y expressions), rounded once to T.
 template <typename T>
 void inst_coef(const Inst &in, SweepCoef<T> *cx, SweepCoef<T> *cy) {
-  const double c1 = in.c1, c2 = in.c2, rho = in.rho;
-  const double irhoc1 = 1.0 / (c1 * rho), irhoc2 = 1.0 / (c2 * rho);
-  const double rhoc1 = c1 * rho, rhoc2 = c2 * rho;
-  const double mu = rho * c2 * c2;
-  const double la = rho * c1 * c1 - 2.0 * mu;
-  const double rhoc3 = rho * c1 * la / (la + 2.0 * mu);
-  for (int a = 0; a < 2; ++a) {
-    const double h = a == 0 ? in.hx : in.hy;
-    const double k1 = in.dt * c1 / h, k2 = in.dt * c2 / h;
-    SweepCoef<T> &c = a == 0 ? *cx : *cy;
-    c.k1 = (T)k1;
-    c.k2 = (T)k2;
-    c.hk1 = (T)((1.0 - k1) * 0.5);
-    c.hk2 = (T)((1.0 - k2) * 0.5);
-    c.irhoc1 = (T)irhoc1;
-    c.irhoc2 = (T)irhoc2;
-    c.rhoc1 = (T)rhoc1;
-    c.rhoc2 = (T)rhoc2;
-    c.rhoc3 = (T)rhoc3;
-  }
+  coef_from_material<T>(in.c1, in.c2, in.rho, in.dt, in.hx, in.hy, cx, cy);
 }
 
 int hcheck(hipError_t e, const char *what) {
@@ -199,6 +128,7 @@ struct BatchImpl {
       h[i].pitch = in.pitch;
       h[i].plane = in.plane;
       inst_coef<T>(in, &h[i].cx, &h[i].cy);
+      h[i].par = 0;
     }
     if (!d_inst && hipMalloc(&d_inst, h.size() * sizeof(InstDesc<T>)) != hipSuccess)
       return set_error(ERR_NOMEM, "batch: descriptor table");
@@ -226,30 +156,17 @@ struct BatchImpl {
     if (rc) return rc;
     ntiles = (int)h.size();
     shape = s;
-    // persistent grid: resident workgroups on the whole device, a multiple of 8
-    int dev = 0, cus = 256, per_cu = 1;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    occupancy(s.fb, &per_cu);
-    long want = (long)cus * std::max(1, per_cu);
-    want = std::min<long>(want, ((long)ntiles + 7) / 8 * 8);
-    blocks = (int)std::max<long>(8, want / 8 * 8);
+    // persistent grid: resident workgroups on the whole device
+    if (elem == 8)
+      blocks = s.fb == 64 ? persistent_blocks<double, 64>(ntiles)
+                          : (s.fb == 128 ? persistent_blocks<double, 128>(ntiles)
+                                         : persistent_blocks<double, 256>(ntiles));
+    else
+      blocks = s.fb == 64 ? persistent_blocks<float, 64>(ntiles)
+                          : (s.fb == 128 ? persistent_blocks<float, 128>(ntiles)
+                                         : persistent_blocks<float, 256>(ntiles));
     drop_graphs();
     return OK;
-  }
-
-  void occupancy(int fb, int *per_cu) {
-    hipError_t e = hipErrorInvalidValue;
-    if (elem == 8) {
-      if (fb == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<double, 64>, 64, 0);
-      if (fb == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<double, 128>, 128, 0);
-      if (fb == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<double, 256>, 256, 0);
-    } else {
-      if (fb == 64) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<float, 64>, 64, 0);
-      if (fb == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<float, 128>, 128, 0);
-      if (fb == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_step_batch<float, 256>, 256, 0);
-    }
-    if (e != hipSuccess || *per_cu < 1) *per_cu = 1;
   }
 
   template <typename T, int FB>

@@ -1,0 +1,530 @@
+// serve.hip -- a serving loop for simulation requests: admission by HBM
+// budget and slots, fields resident on the device, continuous batching of the
+// live requests' steps.
+//
+// A request is one grid (size, material, disk source, step count, FP32 or
+// FP64).  An engine thread runs chunks: it admits queued requests in arrival
+// order while the budget (a device arena sub-allocated first-fit) and the
+// slot count allow, initialises their fields on the device, then launches
+// `chunk` steps in which every live request advances by one step per launch
+// -- one persistent k_step_batch launch per precision per step, FP32 and FP64
+// on their own streams so they overlap.  The chunk ends when the first live
+// request reaches its step count (or after at most `max_chunk` steps); that
+// request completes there, its memory returns to the arena and waiting
+// requests are admitted.  The arithmetic of a request does not depend on
+// which other requests share its launches, so its result is bit-identical
+// alone and in any mix.
+//
+// With plain = 1 the server is the serial baseline: one slot, and each
+// request runs with the single-grid kernel (k_step_xy) and its default tiles.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch_dev.h"
+#include "le2d.h"
+
+namespace le2d {
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over [0, size) in 64 KiB units, coalescing on free.
+class Arena {
+ public:
+  explicit Arena(size_t size) { free_[0] = size; }
+  bool alloc(size_t bytes, size_t *off) {
+    bytes = (bytes + kUnit - 1) / kUnit * kUnit;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes, at = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[at] = rest;
+        used_ += bytes;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = (bytes + kUnit - 1) / kUnit * kUnit;
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  size_t used() const { return used_; }
+
+ private:
+  static constexpr size_t kUnit = 65536;
+  std::map<size_t, size_t> free_;
+  size_t used_ = 0;
+};
+
+enum ReqState { QUEUED = 0, LIVE = 1, DONE = 2, RELEASED = 3 };
+
+struct Req {
+  le2d_request spec;
+  int state = QUEUED;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  size_t off = 0, bytes = 0;
+  long pitch = 0, plane = 0;
+  int cur = 0;
+  int left = 0;
+};
+
+}  // namespace
+
+struct Server {
+  size_t budget;
+  int max_live;
+  int plain;
+  int max_chunk = 16;
+  int fb = 256, seg = 32;
+  char *arena_dev = nullptr;
+  Arena arena;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<long long> queue;
+  std::unordered_map<long long, Req> reqs;
+  std::vector<long long> live;
+  long long next_id = 0;
+  bool stopping = false;
+  bool failed = false;
+  char fail_msg[256] = "";
+  hipStream_t stream[2] = {nullptr, nullptr};  // [0] FP32, [1] FP64
+  void *d_inst[2] = {nullptr, nullptr};
+  TileJob *d_tiles[2] = {nullptr, nullptr};
+  size_t cap_inst[2] = {0, 0}, cap_tiles[2] = {0, 0};
+  int ntiles[2] = {0, 0}, blocks[2] = {8, 8};
+  bool dirty = true;
+  // statistics
+  long long completed = 0;
+  double node_updates = 0, busy_s = 0;
+  size_t peak_bytes = 0;
+  int peak_live = 0;
+  std::thread th;
+
+  Server(size_t b, int ml, int p) : budget(b), max_live(ml), plain(p), arena(b) {}
+
+  static size_t req_bytes(const le2d_request &q) {
+    return 2 * ((grid_bytes(q.real_size, q.nx, q.ny) + 65535) / 65536 * 65536);
+  }
+
+  void fail(const char *what) {
+    std::lock_guard<std::mutex> lk(mu);
+    failed = true;
+    snprintf(fail_msg, sizeof fail_msg, "%s: %s", what, last_error());
+    cv_done.notify_all();
+  }
+
+  // Admit queued requests in order while they fit (caller holds mu).
+  void admit_locked() {
+    while (!queue.empty() && (int)live.size() < max_live) {
+      Req &r = reqs[queue.front()];
+      size_t off = 0;
+      if (!arena.alloc(r.bytes, &off)) break;
+      r.off = off;
+      r.state = LIVE;
+      r.t_admit = now_s();
+      r.cur = 0;
+      r.left = r.spec.steps;
+      live.push_back(queue.front());
+      queue.pop_front();
+      dirty = true;
+      init_fields(r);
+    }
+    peak_bytes = std::max(peak_bytes, arena.used());
+    peak_live = std::max(peak_live, (int)live.size());
+  }
+
+  void init_fields(Req &r) {
+    const le2d_request &q = r.spec;
+    r.pitch = plane_pitch(q.nx, q.real_size);
+    r.plane = (long)(grid_bytes(q.real_size, q.nx, q.ny) / 5 / q.real_size);
+    const int si = q.real_size == 8 ? 1 : 0;
+    char *base = arena_dev + r.off;
+    (void)hipMemsetAsync(base, 0, r.bytes, stream[si]);
+    const long n = (long)q.nx * q.ny;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (q.real_size == 8)
+      hipLaunchKernelGGL(k_set_ball<double>, dim3(nb), dim3(256), 0, stream[si], (double *)base,
+                         q.nx, q.ny, r.pitch, r.plane, q.hx, q.hy, q.cx, q.cy, q.r, q.s);
+    else
+      hipLaunchKernelGGL(k_set_ball<float>, dim3(nb), dim3(256), 0, stream[si], (float *)base,
+                         q.nx, q.ny, r.pitch, r.plane, (float)q.hx, (float)q.hy, (float)q.cx,
+                         (float)q.cy, (float)q.r, (float)q.s);
+  }
+
+  char *buf(const Req &r, int k) { return arena_dev + r.off + k * (r.bytes / 2); }
+
+  template <typename T>
+  int build_tables(int si, const std::vector<Req *> &set) {
+    std::vector<InstDesc<T>> desc(set.size());
+    std::vector<TileJob> tiles;
+    for (size_t i = 0; i < set.size(); ++i) {
+      const Req &r = *set[i];
+      const le2d_request &q = r.spec;
+      InstDesc<T> &d = desc[i];
+      d.buf[0] = reinterpret_cast<T *>(buf(r, 0));
+      d.buf[1] = reinterpret_cast<T *>(buf(r, 1));
+      d.nx = q.nx;
+      d.ny = q.ny;
+      d.pitch = r.pitch;
+      d.plane = r.plane;
+      d.par = r.cur;
+      coef_from_material<T>(q.c1, q.c2, q.rho, q.dt, q.hx, q.hy, &d.cx, &d.cy);
+      const int strips = (q.nx + fb - 5) / (fb - 4);
+      const int sg = std::max(1, std::min(seg, q.ny));
+      for (int j0 = 0; j0 < q.ny; j0 += sg)
+        for (int s = 0; s < strips; ++s) tiles.push_back({(int)i, s, j0, std::min(j0 + sg, q.ny)});
+    }
+    if (desc.size() * sizeof(InstDesc<T>) > cap_inst[si]) {
+      if (d_inst[si]) (void)hipFree(d_inst[si]);
+      cap_inst[si] = std::max<size_t>(desc.size() * sizeof(InstDesc<T>) * 2, 4096);
+      if (hipMalloc(&d_inst[si], cap_inst[si]) != hipSuccess)
+        return set_error(ERR_NOMEM, "server: descriptors");
+    }
+    if (tiles.size() * sizeof(TileJob) > cap_tiles[si]) {
+      if (d_tiles[si]) (void)hipFree(d_tiles[si]);
+      cap_tiles[si] = std::max<size_t>(tiles.size() * sizeof(TileJob) * 2, 65536);
+      if (hipMalloc((void **)&d_tiles[si], cap_tiles[si]) != hipSuccess)
+        return set_error(ERR_NOMEM, "server: tiles");
+    }
+    // tables are read by launches queued on this stream: copy in stream order
+    if (!desc.empty() &&
+        (hipMemcpyAsync(d_inst[si], desc.data(), desc.size() * sizeof(InstDesc<T>),
+                        hipMemcpyHostToDevice, stream[si]) != hipSuccess ||
+         hipMemcpyAsync(d_tiles[si], tiles.data(), tiles.size() * sizeof(TileJob),
+                        hipMemcpyHostToDevice, stream[si]) != hipSuccess ||
+         hipStreamSynchronize(stream[si]) != hipSuccess))
+      return set_error(ERR_DEVICE, "server: tables");
+    ntiles[si] = (int)tiles.size();
+    blocks[si] = fb == 64 ? persistent_blocks<T, 64>(ntiles[si])
+                          : (fb == 128 ? persistent_blocks<T, 128>(ntiles[si])
+                                       : persistent_blocks<T, 256>(ntiles[si]));
+    return OK;
+  }
+
+  template <typename T>
+  void launch_batch(int si, int step_parity) {
+    const InstDesc<T> *d = static_cast<const InstDesc<T> *>(d_inst[si]);
+    if (fb == 64)
+      hipLaunchKernelGGL((k_step_batch<T, 64>), dim3(blocks[si]), dim3(64), 0, stream[si], d,
+                         d_tiles[si], ntiles[si], step_parity);
+    else if (fb == 128)
+      hipLaunchKernelGGL((k_step_batch<T, 128>), dim3(blocks[si]), dim3(128), 0, stream[si], d,
+                         d_tiles[si], ntiles[si], step_parity);
+    else
+      hipLaunchKernelGGL((k_step_batch<T, 256>), dim3(blocks[si]), dim3(256), 0, stream[si], d,
+                         d_tiles[si], ntiles[si], step_parity);
+  }
+
+  // The serial baseline's step: the single-grid kernel on the one live request.
+  int plain_step(Req &r) {
+    const le2d_request &q = r.spec;
+    DevGrid g;
+    g.elem = q.real_size;
+    g.nx = q.nx;
+    g.ny = q.ny;
+    g.pitch = r.pitch;
+    g.plane = r.plane;
+    g.buf[0] = buf(r, 0);
+    g.buf[1] = buf(r, 1);
+    g.cur = r.cur;
+    g.stream = stream[q.real_size == 8 ? 1 : 0];
+    g.tiles = default_tiles(g.elem, g.nx, g.ny);
+    if (q.real_size == 8) {
+      SweepCoef<double> cx, cy;
+      coef_from_material<double>(q.c1, q.c2, q.rho, q.dt, q.hx, q.hy, &cx, &cy);
+      return step_xy<double>(&g, cx, cy, EDGE_CLAMP);
+    }
+    SweepCoef<float> cx, cy;
+    coef_from_material<float>(q.c1, q.c2, q.rho, q.dt, q.hx, q.hy, &cx, &cy);
+    return step_xy<float>(&g, cx, cy, EDGE_CLAMP);
+  }
+
+  void run() {
+    if (hipSetDevice(0) != hipSuccess) {
+      fail("server: device");
+      return;
+    }
+    for (;;) {
+      std::vector<Req *> set[2];
+      int chunk = 0;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        admit_locked();
+        cv_work.wait(lk, [&] { return stopping || !live.empty(); });
+        if (live.empty() && stopping) return;
+        chunk = max_chunk;
+        for (long long id : live) {
+          Req &r = reqs[id];
+          chunk = std::min(chunk, r.left);
+          set[r.spec.real_size == 8 ? 1 : 0].push_back(&r);
+        }
+      }
+      const double t0 = now_s();
+      int rc = OK;
+      if (plain) {
+        Req &r = set[0].empty() ? *set[1][0] : *set[0][0];
+        for (int s = 0; s < chunk && !rc; ++s) {
+          rc = plain_step(r);  // reads buf[r.cur], writes the other
+          r.cur ^= 1;
+        }
+      } else {
+        if (dirty) {
+          if (!set[0].empty()) rc = build_tables<float>(0, set[0]);
+          if (!rc && !set[1].empty()) rc = build_tables<double>(1, set[1]);
+          dirty = false;
+        }
+        for (int s = 0; s < chunk && !rc; ++s) {
+          if (!set[0].empty()) launch_batch<float>(0, s & 1);
+          if (!set[1].empty()) launch_batch<double>(1, s & 1);
+        }
+        if (!rc && hipGetLastError() != hipSuccess) rc = set_error(ERR_DEVICE, "server: launch");
+      }
+      for (int si = 0; si < 2 && !rc; ++si)
+        if (hipStreamSynchronize(stream[si]) != hipSuccess) rc = set_error(ERR_DEVICE, "server: sync");
+      if (rc) {
+        fail("server step");
+        return;
+      }
+      const double t1 = now_s();
+      std::lock_guard<std::mutex> lk(mu);
+      busy_s += t1 - t0;
+      std::vector<long long> still;
+      for (long long id : live) {
+        Req &r = reqs[id];
+        if (!plain) r.cur ^= chunk & 1;
+        r.left -= chunk;
+        node_updates += (double)r.spec.nx * r.spec.ny * chunk;
+        if (r.left == 0) {
+          r.state = DONE;
+          r.t_done = t1;
+          ++completed;
+          dirty = true;
+          if (!r.spec.keep_result) {
+            arena.release(r.off, r.bytes);
+            r.state = RELEASED;
+          }
+        } else {
+          still.push_back(id);
+        }
+      }
+      if (still.size() != live.size()) {
+        live.swap(still);
+        dirty = true;
+        cv_done.notify_all();
+      }
+      if (!plain) {
+        // tables hold each request's parity at build time: rebuild after an
+        // odd chunk so par stays the current buffer
+        if (chunk & 1) dirty = true;
+      }
+    }
+  }
+};
+
+}  // namespace le2d
+
+using le2d::Server;
+
+struct le2d_server {
+  Server *s;
+};
+
+extern "C" {
+
+le2d_server *le2d_server_create(size_t hbm_budget, int max_live, int plain) {
+  le2d::require_device();
+  if (hbm_budget < (1u << 20) || max_live < 1) {
+    le2d::set_error(le2d::ERR_ARG, "server: budget %zu, slots %d", hbm_budget, max_live);
+    return nullptr;
+  }
+  Server *s = new Server(hbm_budget, plain ? 1 : max_live, plain);
+  if (hipMalloc((void **)&s->arena_dev, hbm_budget) != hipSuccess) {
+    le2d::set_error(le2d::ERR_NOMEM, "server: arena of %zu bytes", hbm_budget);
+    delete s;
+    return nullptr;
+  }
+  for (int k = 0; k < 2; ++k)
+    if (hipStreamCreateWithFlags(&s->stream[k], hipStreamNonBlocking) != hipSuccess) {
+      le2d::set_error(le2d::ERR_DEVICE, "server: stream");
+      (void)hipFree(s->arena_dev);
+      delete s;
+      return nullptr;
+    }
+  s->th = std::thread([s] { s->run(); });
+  le2d_server *h = new le2d_server;
+  h->s = s;
+  return h;
+}
+
+int le2d_server_set_shape(le2d_server *h, int fb, int seg, int max_chunk) {
+  if (!h || (fb != 64 && fb != 128 && fb != 256) || seg < 1 || max_chunk < 1)
+    return le2d::set_error(le2d::ERR_ARG, "server: shape");
+  std::lock_guard<std::mutex> lk(h->s->mu);
+  h->s->fb = fb;
+  h->s->seg = seg;
+  h->s->max_chunk = max_chunk;
+  h->s->dirty = true;
+  return le2d::OK;
+}
+
+long long le2d_server_submit(le2d_server *h, const le2d_request *q) {
+  if (!h || !q) return -le2d::ERR_ARG;
+  if ((q->real_size != 4 && q->real_size != 8) || q->nx < 3 || q->ny < 3 || q->steps < 1 ||
+      !(q->dt > 0) || !(q->hx > 0) || !(q->hy > 0) || !(q->c1 > 0) || !(q->c2 > 0) ||
+      !(q->rho > 0))
+    return -le2d::set_error(le2d::ERR_ARG, "server: bad request");
+  Server *s = h->s;
+  const size_t bytes = Server::req_bytes(*q);
+  if (bytes > s->budget)
+    return -le2d::set_error(le2d::ERR_BUDGET, "server: request needs %zu bytes, budget %zu", bytes,
+                            s->budget);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->failed) return -le2d::set_error(le2d::ERR_DEVICE, "%s", s->fail_msg);
+  const long long id = s->next_id++;
+  le2d::Req &r = s->reqs[id];
+  r.spec = *q;
+  r.bytes = bytes;
+  r.t_submit = le2d::now_s();
+  s->queue.push_back(id);
+  s->cv_work.notify_one();
+  return id;
+}
+
+int le2d_server_wait(le2d_server *h, long long id, double timeout_s) {
+  if (!h) return le2d::ERR_ARG;
+  Server *s = h->s;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return le2d::set_error(le2d::ERR_ARG, "server: no request %lld", id);
+  const auto done = [&] { return s->failed || it->second.state >= le2d::DONE; };
+  if (timeout_s < 0)
+    s->cv_done.wait(lk, done);
+  else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done))
+    return le2d::set_error(le2d::ERR_STATE, "server: request %lld not done", id);
+  if (s->failed) return le2d::set_error(le2d::ERR_DEVICE, "%s", s->fail_msg);
+  return le2d::OK;
+}
+
+int le2d_server_drain(le2d_server *h, double timeout_s) {
+  if (!h) return le2d::ERR_ARG;
+  Server *s = h->s;
+  std::unique_lock<std::mutex> lk(s->mu);
+  const auto idle = [&] { return s->failed || (s->queue.empty() && s->live.empty()); };
+  if (timeout_s < 0)
+    s->cv_done.wait(lk, idle);
+  else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), idle))
+    return le2d::set_error(le2d::ERR_STATE, "server: not drained");
+  return s->failed ? le2d::set_error(le2d::ERR_DEVICE, "%s", s->fail_msg) : le2d::OK;
+}
+
+int le2d_server_result(le2d_server *h, long long id, void *aos) {
+  if (!h) return le2d::ERR_ARG;
+  Server *s = h->s;
+  le2d::Req r;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end() || it->second.state != le2d::DONE || !it->second.spec.keep_result)
+      return le2d::set_error(le2d::ERR_STATE, "server: no kept result for %lld", id);
+    r = it->second;
+  }
+  le2d::DevGrid g;
+  g.elem = r.spec.real_size;
+  g.nx = r.spec.nx;
+  g.ny = r.spec.ny;
+  g.pitch = r.pitch;
+  g.plane = r.plane;
+  g.buf[0] = s->buf(r, 0);
+  g.buf[1] = s->buf(r, 1);
+  g.cur = r.cur;
+  hipStream_t st;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+    return le2d::set_error(le2d::ERR_DEVICE, "server: result stream");
+  g.stream = st;
+  int rc = le2d::grid_download(&g, aos, 0);
+  (void)hipStreamDestroy(st);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(s->mu);
+  le2d::Req &rr = s->reqs[id];
+  s->arena.release(rr.off, rr.bytes);
+  rr.state = le2d::RELEASED;
+  s->cv_work.notify_one();
+  return le2d::OK;
+}
+
+int le2d_server_times(le2d_server *h, long long id, double *t_submit, double *t_admit,
+                      double *t_done) {
+  if (!h) return le2d::ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->s->mu);
+  auto it = h->s->reqs.find(id);
+  if (it == h->s->reqs.end()) return le2d::set_error(le2d::ERR_ARG, "server: no request %lld", id);
+  *t_submit = it->second.t_submit;
+  *t_admit = it->second.t_admit;
+  *t_done = it->second.t_done;
+  return it->second.state >= le2d::DONE ? le2d::OK : le2d::ERR_STATE;
+}
+
+int le2d_server_get_stats(le2d_server *h, le2d_server_stats *out) {
+  if (!h || !out) return le2d::ERR_ARG;
+  Server *s = h->s;
+  std::lock_guard<std::mutex> lk(s->mu);
+  out->completed = s->completed;
+  out->queued = (long long)s->queue.size();
+  out->live = (int)s->live.size();
+  out->peak_live = s->peak_live;
+  out->node_updates = s->node_updates;
+  out->busy_seconds = s->busy_s;
+  out->bytes_in_use = s->arena.used();
+  out->peak_bytes = s->peak_bytes;
+  return le2d::OK;
+}
+
+void le2d_server_destroy(le2d_server *h) {
+  if (!h) return;
+  Server *s = h->s;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+    s->queue.clear();
+    s->cv_work.notify_all();
+  }
+  if (s->th.joinable()) s->th.join();
+  for (int k = 0; k < 2; ++k) {
+    if (s->stream[k]) (void)hipStreamSynchronize(s->stream[k]);
+    if (s->d_inst[k]) (void)hipFree(s->d_inst[k]);
+    if (s->d_tiles[k]) (void)hipFree(s->d_tiles[k]);
+    if (s->stream[k]) (void)hipStreamDestroy(s->stream[k]);
+  }
+  if (s->arena_dev) (void)hipFree(s->arena_dev);
+  delete s;
+  delete h;
+}
+
+}  // extern "C"

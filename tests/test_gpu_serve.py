@@ -1,0 +1,102 @@
+"""Serving loop (le2d_server_*): admission by HBM budget and slots, continuous
+batching; each result is bit-identical alone and in a mix of other requests,
+and FP64 results are bit-identical to the oracle."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import le2d_amd
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+OD = Oracle(np.float64)
+GB = 1 << 30
+
+REQ = dict(nx=300, ny=257, steps=37, dt=0.12, h=(1.0, 0.9), mat=(2.7, 1.3, 1.9),
+           disk=(120.5, 101.0, 20.0, 1.0))
+
+
+def _others(rng, k):
+    out = []
+    for _ in range(k):
+        nx, ny = (int(rng.integers(16, 400)) for _ in range(2))
+        c1 = float(rng.uniform(1, 4))
+        out.append(dict(nx=nx, ny=ny, steps=int(rng.integers(1, 60)),
+                        dtype=np.float32 if rng.random() < 0.7 else np.float64,
+                        dt=0.45 / c1, mat=(c1, c1 / float(rng.uniform(1.5, 2.2)),
+                                           float(rng.uniform(1, 3))),
+                        disk=(nx / 3, ny / 2, float(rng.uniform(2, 10)), 1.0)))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_alone_equals_mix(dtype):
+    with le2d_amd.Server(1 * GB, 64) as s:
+        i = s.submit(**REQ, dtype=dtype, keep=True)
+        s.wait(i)
+        alone = s.result(i)
+    rng = np.random.default_rng(9)
+    with le2d_amd.Server(1 * GB, 64) as s:
+        s.set_shape(128, 16, 5)
+        ids = [s.submit(**o) for o in _others(rng, 10)]
+        i = s.submit(**REQ, dtype=dtype, keep=True)
+        done = []
+
+        def late():  # requests arriving while the probe is in flight
+            for o in _others(rng, 12):
+                done.append(s.submit(**o))
+                time.sleep(0.002)
+        th = threading.Thread(target=late)
+        th.start()
+        th.join()
+        s.wait(i)
+        mixed = s.result(i)
+        s.drain()
+        st = s.stats()
+        assert st["completed"] == len(ids) + len(done) + 1
+        assert st["peak_live"] > 1
+    assert np.array_equal(alone, mixed)
+
+
+def test_fp64_request_matches_oracle():
+    with le2d_amd.Server(1 * GB, 8) as s:
+        i = s.submit(**REQ, dtype=np.float64, keep=True)
+        s.wait(i)
+        got = s.result(i)
+    ref = np.zeros((257, 300, 5))
+    OD.set_ball(ref, REQ["h"], REQ["disk"][:2], REQ["disk"][2], REQ["disk"][3])
+    OD.run(ref, REQ["dt"], REQ["h"], REQ["mat"], REQ["steps"])
+    assert np.array_equal(got, ref)
+
+
+def test_serial_plain_server_equals_batched():
+    res = []
+    for plain in (False, True):
+        with le2d_amd.Server(1 * GB, 16, plain=plain) as s:
+            ids = [s.submit(**REQ, dtype=dt, keep=True) for dt in (np.float32, np.float64)]
+            s.drain()
+            res.append([s.result(i) for i in ids])
+            if plain:
+                assert s.stats()["peak_live"] == 1
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
+
+
+def test_budget_limits_admission_and_rejects_oversize():
+    # a 512x512 FP64 request holds two states of ~10.5 MB each
+    budget = 48 << 20
+    with le2d_amd.Server(budget, 64) as s:
+        ids = [s.submit(nx=512, ny=512, steps=20, dtype=np.float64) for _ in range(6)]
+        s.drain()
+        st = s.stats()
+        assert st["completed"] == 6
+        assert st["peak_live"] == 2
+        assert st["peak_bytes"] <= budget
+        assert st["bytes_in_use"] == 0
+        with pytest.raises(le2d_amd.LE2DError):
+            s.submit(nx=4096, ny=4096, steps=1, dtype=np.float64)
+        for i in ids:
+            ts, ta, td = s.times(i)
+            assert ts <= ta <= td
