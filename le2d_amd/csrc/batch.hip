@@ -145,7 +145,8 @@ struct BatchImpl {
       const int strips = (in.nx + s.fb - 5) / (s.fb - 4);
       const int seg = std::max(1, std::min(s.seg, in.ny));
       for (int j0 = 0; j0 < in.ny; j0 += seg)
-        for (int st = 0; st < strips; ++st) h.push_back({(int)i, st, j0, std::min(j0 + seg, in.ny)});
+        for (int st = 0; st < strips; ++st)
+          h.push_back({(int)i, st, j0, std::min(j0 + seg, in.ny), j0 / seg});
     }
     if (d_tiles) (void)hipFree(d_tiles);
     d_tiles = nullptr;

@@ -23,6 +23,7 @@ struct InstDesc {
 
 struct TileJob {
   int inst, strip, j0, j1;
+  int seg;  // segment index within the instance (sets the march direction)
 };
 
 // Persistent over the tile table.  Workgroup b runs on XCD b % 8 (dispatch
@@ -41,8 +42,8 @@ __global__ __launch_bounds__(FB) void k_step_batch(const InstDesc<T> *__restrict
     const TileJob tj = tiles[k];
     const InstDesc<T> &d = inst[tj.inst];
     const int p = d.par ^ step_parity;
-    march_tile<T, EDGE_CLAMP, FB>(d.buf[p], d.buf[p ^ 1], d.nx, d.ny, d.pitch, d.plane, tj.strip,
-                                  tj.j0, tj.j1, d.cx, d.cy, sw, sf);
+    march_tile_alt<T, EDGE_CLAMP, FB>(d.buf[p], d.buf[p ^ 1], d.nx, d.ny, d.pitch, d.plane,
+                                      tj.strip, tj.j0, tj.j1, tj.seg, d.cx, d.cy, sw, sf);
   }
 }
 

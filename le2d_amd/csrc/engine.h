@@ -88,6 +88,7 @@ Tiles default_tiles(int elem, int nx, int ny);
 // materials (c1, c2, rho); see media.hip for the scheme.
 struct MatTile {
   int strip, j0, j1, mat;  // mat >= 0: the tile's one material; -1: mixed
+  int seg;                 // segment index (sets the march direction)
 };
 
 struct MatPlane {

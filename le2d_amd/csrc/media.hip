@@ -26,7 +26,8 @@ __global__ __launch_bounds__(FB) void k_step_mat_homog(const T *__restrict__ src
   __shared__ W4<T> sf[FB];
   const MatTile tj = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const SweepCoef<T> cx = cxt[tj.mat], cy = cyt[tj.mat];
-  march_tile<T, EDGE, FB>(src, dst, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cx, cy, sw, sf);
+  march_tile_alt<T, EDGE, FB>(src, dst, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, tj.seg, cx,
+                              cy, sw, sf);
 }
 
 template <typename T, int EDGE, int FB>
@@ -273,7 +274,7 @@ static int mat_tiles(DevGrid *g, MatPlane *m, int edge) {
             break;
           }
       }
-      (one ? homog : mixed).push_back({s, j0, j1, one ? (int)first : -1});
+      (one ? homog : mixed).push_back({s, j0, j1, one ? (int)first : -1, j0 / seg});
     }
   }
   for (MatTile **d : {&m->d_homog, &m->d_mixed})

@@ -199,7 +199,8 @@ struct Server {
       const int strips = (q.nx + fb - 5) / (fb - 4);
       const int sg = std::max(1, std::min(seg, q.ny));
       for (int j0 = 0; j0 < q.ny; j0 += sg)
-        for (int s = 0; s < strips; ++s) tiles.push_back({(int)i, s, j0, std::min(j0 + sg, q.ny)});
+        for (int s = 0; s < strips; ++s)
+          tiles.push_back({(int)i, s, j0, std::min(j0 + sg, q.ny), j0 / sg});
     }
     if (desc.size() * sizeof(InstDesc<T>) > cap_inst[si]) {
       if (d_inst[si]) (void)hipFree(d_inst[si]);

@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void k_sweep_y(const T *__restrict__ src, T *_
 // ---------------------------------------------------------------------------
 // Fused step over one grid (layout and march: march_tile in sweep_dev.h).
 template <typename T, int EDGE, int FB>
-__global__ __launch_bounds__(FB) void k_step_xy(const T *__restrict__ src, T *__restrict__ dst,
+__global__ __launch_bounds__(FB, 3) void k_step_xy(const T *__restrict__ src, T *__restrict__ dst,
                                                 int nx, int ny, long pitch, long plane, int seg,
                                                 int nstrips, SweepCoef<T> cx, SweepCoef<T> cy) {
   __shared__ W4<T> sw[FB];
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(FB) void k_step_xy(const T *__restrict__ src, T *__
   const int strip = tile % nstrips, segi = tile / nstrips;
   const int j0 = segi * seg;
   const int j1 = min(j0 + seg, ny);
-  march_tile<T, EDGE, FB>(src, dst, nx, ny, pitch, plane, strip, j0, j1, cx, cy, sw, sf);
+  march_tile_alt<T, EDGE, FB>(src, dst, nx, ny, pitch, plane, strip, j0, j1, segi, cx, cy, sw, sf);
 }
 
 // ---------------------------------------------------------------------------

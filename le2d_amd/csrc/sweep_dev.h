@@ -234,8 +234,14 @@ __device__ __forceinline__ Node<T> window_xsweep(Node<T> u, W4<T> *sw, W4<T> *sf
 
 // One tile of the fused step: the window of strip `strip` (lane t reads
 // column strip*(FB-4)-2+t, clamped; lanes 2..FB-3 write) over output rows
-// [j0, j1).  Used by the single-grid kernel and the batched kernel alike.
-template <typename T, int EDGE, int FB>
+// [j0, j1), marched downwards (DIR = +1, j increasing) or upwards (DIR = -1).
+// Callers alternate the direction between neighbouring segments of a strip,
+// so the two segments that read the halo rows around a shared boundary do so
+// at about the same time (both start there, or both end there) and the
+// second read hits L2.  Each face is the same operands and operations in
+// either direction, so the result does not depend on DIR.  Used by the
+// single-grid, material and batched kernels alike.
+template <typename T, int EDGE, int FB, int DIR>
 __device__ __forceinline__ void march_tile(const T *__restrict__ src, T *__restrict__ dst, int nx,
                                            int ny, long pitch, long plane, int strip, int j0,
                                            int j1, const SweepCoef<T> &cx,
@@ -246,35 +252,60 @@ __device__ __forceinline__ void march_tile(const T *__restrict__ src, T *__restr
   const unsigned woff = (unsigned)max(x, 0) * (unsigned)sizeof(T);
   const bool writer = t >= 2 && t <= FB - 3 && x < nx;
 #define ROW(p) load_row_node(src + (long)ypos<EDGE>((p), ny) * pitch, plane, boff)
-
-  // prologue: positions j0-2 .. j0+2 (x-swept), the raw row of j0+3 in flight
-  const W4<T> wm2 = inv_y(window_xsweep<T, FB>(ROW(j0 - 2), sw, sf, t, cx), cy);
-  W4<T> wm1 = inv_y(window_xsweep<T, FB>(ROW(j0 - 1), sw, sf, t, cx), cy);
-  Node<T> A = window_xsweep<T, FB>(ROW(j0), sw, sf, t, cx);
-  Node<T> B = window_xsweep<T, FB>(ROW(j0 + 1), sw, sf, t, cx);
-  Node<T> C = window_xsweep<T, FB>(ROW(j0 + 2), sw, sf, t, cx);
-  Node<T> N0 = ROW(j0 + 3), N1;
-  W4<T> F0 = faces(wm2, wm1, inv_y(A, cy), inv_y(B, cy), cy), F1;  // face j0-1/2
-
-  // Step j.  Slots P0/P1/P2 hold the x-swept nodes of positions j, j+1, j+2
+#define XS(n) window_xsweep<T, FB>((n), sw, sf, t, cx)
+  // Slots P0/P1/P2 hold the x-swept nodes of positions j, j+DIR, j+2*DIR
   // (their y-invariants are re-formed when needed, 6 operations each, rather
-  // than held), FI is the inflow face j-1/2 and NC the raw row of position
-  // j+3.  Row j is written, then position j+3 is x-swept into P0 while NN
-  // receives the raw row of position j+4.  Unrolled six times (slot period 3,
-  // face and prefetch period 2) so no register is copied between steps.
+  // than held), FI is the face on the side already passed and NC the raw row
+  // of position j+3*DIR.  Row j is written, then position j+3*DIR is x-swept
+  // into P0 while NN receives the raw row of position j+4*DIR.  Unrolled six
+  // times (slot period 3, face and prefetch period 2) so no register is
+  // copied between steps.  wb keeps the invariants of position j-DIR (the
+  // components its faces still read).
+  Node<T> A, B, C, N0, N1;
+  W4<T> F0, F1, wb;
+  int j;
+  if constexpr (DIR > 0) {
+    const W4<T> w2 = inv_y(XS(ROW(j0 - 2)), cy);
+    wb = inv_y(XS(ROW(j0 - 1)), cy);
+    A = XS(ROW(j0));
+    B = XS(ROW(j0 + 1));
+    C = XS(ROW(j0 + 2));
+    N0 = ROW(j0 + 3);
+    F0 = faces(w2, wb, inv_y(A, cy), inv_y(B, cy), cy);  // face j0-1 | j0
+    j = j0;
+  } else {
+    const W4<T> w2 = inv_y(XS(ROW(j1 + 1)), cy);
+    wb = inv_y(XS(ROW(j1)), cy);
+    A = XS(ROW(j1 - 1));
+    B = XS(ROW(j1 - 2));
+    C = XS(ROW(j1 - 3));
+    N0 = ROW(j1 - 4);
+    F0 = faces(inv_y(B, cy), inv_y(A, cy), wb, w2, cy);  // face j1-1 | j1
+    j = j1 - 1;
+  }
 #define STEP(P0, P1, P2, FI, FO, NC, NN)                                     \
   {                                                                           \
-    NN = ROW(j + 4);                                                          \
+    NN = ROW(j + 4 * DIR);                                                    \
     const W4<T> w0 = inv_y(P0, cy);                                           \
-    FO = faces(wm1, w0, inv_y(P1, cy), inv_y(P2, cy), cy);                    \
-    apply(P0.vy, P0.vx, P0.syy, P0.sxx, P0.sxy, w0, FI, FO, cy);              \
+    if constexpr (DIR > 0) {                                                  \
+      FO = faces(wb, w0, inv_y(P1, cy), inv_y(P2, cy), cy); /* j | j+1 */     \
+      apply(P0.vy, P0.vx, P0.syy, P0.sxx, P0.sxy, w0, FI, FO, cy);            \
+    } else {                                                                  \
+      FO = faces(inv_y(P2, cy), inv_y(P1, cy), w0, wb, cy); /* j-1 | j */     \
+      apply(P0.vy, P0.vx, P0.syy, P0.sxx, P0.sxy, w0, FO, FI, cy);            \
+    }                                                                         \
     if (writer) store_row_node(dst + (long)j * pitch, plane, woff, P0);      \
-    if (++j >= j1) break;                                                     \
-    wm1.a = w0.a;                                                             \
-    wm1.c = w0.c;                                                             \
-    P0 = window_xsweep<T, FB>(NC, sw, sf, t, cx);                             \
+    j += DIR;                                                                 \
+    if (DIR > 0 ? j >= j1 : j < j0) break;                                    \
+    if constexpr (DIR > 0) {                                                  \
+      wb.a = w0.a;                                                            \
+      wb.c = w0.c;                                                            \
+    } else {                                                                  \
+      wb.b = w0.b;                                                            \
+      wb.d = w0.d;                                                            \
+    }                                                                         \
+    P0 = XS(NC);                                                              \
   }
-  int j = j0;
   for (;;) {
     STEP(A, B, C, F0, F1, N0, N1)
     STEP(B, C, A, F1, F0, N1, N0)
@@ -284,7 +315,21 @@ __device__ __forceinline__ void march_tile(const T *__restrict__ src, T *__restr
     STEP(C, A, B, F1, F0, N1, N0)
   }
 #undef STEP
+#undef XS
 #undef ROW
+}
+
+// Direction by segment index: even segments march down, odd ones up.
+template <typename T, int EDGE, int FB>
+__device__ __forceinline__ void march_tile_alt(const T *__restrict__ src, T *__restrict__ dst,
+                                               int nx, int ny, long pitch, long plane, int strip,
+                                               int j0, int j1, int segi,
+                                               const SweepCoef<T> &cx, const SweepCoef<T> &cy,
+                                               W4<T> *sw, W4<T> *sf) {
+  if (segi & 1)
+    march_tile<T, EDGE, FB, -1>(src, dst, nx, ny, pitch, plane, strip, j0, j1, cx, cy, sw, sf);
+  else
+    march_tile<T, EDGE, FB, 1>(src, dst, nx, ny, pitch, plane, strip, j0, j1, cx, cy, sw, sf);
 }
 
 // ---------------------------------------------------------------------------
