@@ -303,7 +303,7 @@ def serve_report(le2d_amd, window_s=1.0, budget_gb=200.0, slots=1024):
     log(f"serving {len(reqs)} requests")
     with le2d_amd.Server(budget, slots) as s:
         ids = submit_all(s, True)
-        s.drain()
+        s.drain(300)
         times = [s.times(i) for i in ids]
         st = s.stats()
     t_first = min(t[0] for t in times)
@@ -320,7 +320,7 @@ def serve_report(le2d_amd, window_s=1.0, budget_gb=200.0, slots=1024):
     log("serial baseline")
     with le2d_amd.Server(budget, 1, plain=True) as s:
         ids = submit_all(s, False)
-        s.drain()
+        s.drain(300)
         # the engine admits request k+1 when k completes: its service time
         # (initialisation and steps) is the gap between completions
         done = [s.times(i)[2] for i in ids]

@@ -277,9 +277,15 @@ struct Server {
       int chunk = 0;
       {
         std::unique_lock<std::mutex> lk(mu);
-        admit_locked();
-        cv_work.wait(lk, [&] { return stopping || !live.empty(); });
-        if (live.empty() && stopping) return;
+        // Admission runs on every wake: a submit (or a released result) that
+        // arrives while nothing is live only queues, and would otherwise sit
+        // there until the next unrelated wake.
+        for (;;) {
+          if (stopping) return;
+          admit_locked();
+          if (!live.empty()) break;
+          cv_work.wait(lk);
+        }
         chunk = max_chunk;
         for (long long id : live) {
           Req &r = reqs[id];
@@ -424,7 +430,9 @@ int le2d_server_wait(le2d_server *h, long long id, double timeout_s) {
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return le2d::set_error(le2d::ERR_ARG, "server: no request %lld", id);
-  const auto done = [&] { return s->failed || it->second.state >= le2d::DONE; };
+  // submits insert while this waits: a rehash moves iterators, not elements
+  const le2d::Req *r = &it->second;
+  const auto done = [&] { return s->failed || r->state >= le2d::DONE; };
   if (timeout_s < 0)
     s->cv_done.wait(lk, done);
   else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done))

@@ -35,7 +35,7 @@ def _others(rng, k):
 def test_alone_equals_mix(dtype):
     with le2d_amd.Server(1 * GB, 64) as s:
         i = s.submit(**REQ, dtype=dtype, keep=True)
-        s.wait(i)
+        s.wait(i, 60)
         alone = s.result(i)
     rng = np.random.default_rng(9)
     with le2d_amd.Server(1 * GB, 64) as s:
@@ -51,9 +51,9 @@ def test_alone_equals_mix(dtype):
         th = threading.Thread(target=late)
         th.start()
         th.join()
-        s.wait(i)
+        s.wait(i, 60)
         mixed = s.result(i)
-        s.drain()
+        s.drain(60)
         st = s.stats()
         assert st["completed"] == len(ids) + len(done) + 1
         assert st["peak_live"] > 1
@@ -63,7 +63,7 @@ def test_alone_equals_mix(dtype):
 def test_fp64_request_matches_oracle():
     with le2d_amd.Server(1 * GB, 8) as s:
         i = s.submit(**REQ, dtype=np.float64, keep=True)
-        s.wait(i)
+        s.wait(i, 60)
         got = s.result(i)
     ref = np.zeros((257, 300, 5))
     OD.set_ball(ref, REQ["h"], REQ["disk"][:2], REQ["disk"][2], REQ["disk"][3])
@@ -76,7 +76,7 @@ def test_serial_plain_server_equals_batched():
     for plain in (False, True):
         with le2d_amd.Server(1 * GB, 16, plain=plain) as s:
             ids = [s.submit(**REQ, dtype=dt, keep=True) for dt in (np.float32, np.float64)]
-            s.drain()
+            s.drain(60)
             res.append([s.result(i) for i in ids])
             if plain:
                 assert s.stats()["peak_live"] == 1
@@ -89,7 +89,7 @@ def test_budget_limits_admission_and_rejects_oversize():
     budget = 48 << 20
     with le2d_amd.Server(budget, 64) as s:
         ids = [s.submit(nx=512, ny=512, steps=20, dtype=np.float64) for _ in range(6)]
-        s.drain()
+        s.drain(60)
         st = s.stats()
         assert st["completed"] == 6
         assert st["peak_live"] == 2
