@@ -213,6 +213,8 @@ int le2d_server_result(le2d_server *srv, long long id, void *aos);
 int le2d_server_times(le2d_server *srv, long long id, double *t_submit, double *t_admit,
                       double *t_done);
 int le2d_server_get_stats(le2d_server *srv, le2d_server_stats *out);
+/* Stops the engine after its current chunk; requests still queued or live
+ * are abandoned.  No other call on `srv` may be in progress or follow. */
 void le2d_server_destroy(le2d_server *srv);
 
 #ifdef __cplusplus
