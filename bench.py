@@ -100,6 +100,23 @@ def rel_errors(norms):
     return {"l2_rel": e2 / r2 if r2 > 0 else None, "linf_rel": einf / rinf if rinf > 0 else None}
 
 
+def reference_case_report(le2d_amd, n=512, steps=100):
+    """The reference driver's own case (512x512, h = 1, material (2, 1, 1),
+    dt = 0.25, disk r = 10 at the centre, 100 steps): FP32 against FP64."""
+    tasks = {}
+    for dt in (np.float64, np.float32):
+        t = le2d_amd.Task(n, n, 0.25, (1.0, 1.0), (2.0, 1.0, 1.0), dtype=dt)
+        t.set_ball((float(n // 2), float(n // 2)), 10.0, 1.0)
+        t.step_n(steps)
+        tasks[dt] = t
+    errs = rel_errors(le2d_amd.diff_norms(tasks[np.float32].device_view(),
+                                          tasks[np.float64].device_view()))
+    for t in tasks.values():
+        t.close()
+    return {"workload": f"{n}x{n}, disk r=10 at the centre, c1=2 c2=1 rho=1, dt=0.25, "
+                        f"{steps} steps", "fp32_vs_fp64": errs}
+
+
 def cpu_baseline(n, budget_s=12.0):
     """Reference simple/ path (oracle/_ref, compiled from the reference's own
     sources) if present, else the oracle port; FP64, same grid and disk, a
@@ -287,25 +304,44 @@ def serve_report(le2d_amd, window_s=1.0, budget_gb=200.0, slots=1024):
                        f"U[200, 2000], 70% FP32 / 30% FP64, HBM budget {budget_gb:.0f} GB"}
     budget = int(budget_gb * (1 << 30))
 
-    def submit_all(s, realtime):
+    # the FP32 results of a few (the cheapest) FP32 requests are kept from the
+    # mix and compared with the same requests run in FP64
+    f32 = [k for k, q in enumerate(reqs) if q["dtype"] == np.float32]
+    probe = sorted(f32, key=lambda k: reqs[k]["nx"] * reqs[k]["ny"] * reqs[k]["steps"])[:4]
+
+    def submit_all(s, realtime, keep=()):
         ids = []
         t0 = time.perf_counter()
-        for q in reqs:
+        for k, q in enumerate(reqs):
             if realtime:
                 dt = q["at"] - (time.perf_counter() - t0)
                 if dt > 0:
                     time.sleep(dt)
             ids.append(s.submit(q["nx"], q["ny"], q["steps"], q["dtype"], q["dt"], (1.0, 1.0),
-                                q["mat"], q["disk"]))
+                                q["mat"], q["disk"], keep=k in keep))
         return ids
 
     # continuous batching, arrivals in real time
     log(f"serving {len(reqs)} requests")
     with le2d_amd.Server(budget, slots) as s:
-        ids = submit_all(s, True)
+        ids = submit_all(s, True, keep=probe)
         s.drain(300)
         times = [s.times(i) for i in ids]
         st = s.stats()
+        mixed = {k: s.result(ids[k]) for k in probe}
+    errs = []
+    with le2d_amd.Server(budget, slots) as s:
+        twin = {k: s.submit(reqs[k]["nx"], reqs[k]["ny"], reqs[k]["steps"], np.float64,
+                            reqs[k]["dt"], (1.0, 1.0), reqs[k]["mat"], reqs[k]["disk"], keep=True)
+                for k in probe}
+        s.drain(300)
+        for k in probe:
+            ref = s.result(twin[k])
+            d = mixed[k].astype(np.float64) - ref
+            errs.append({"nx": reqs[k]["nx"], "ny": reqs[k]["ny"], "steps": reqs[k]["steps"],
+                         "l2_rel": float(np.sqrt((d * d).sum() / (ref * ref).sum())),
+                         "linf_rel": float(np.abs(d).max() / np.abs(ref).max())})
+    out["fp32_vs_fp64"] = errs
     t_first = min(t[0] for t in times)
     span = max(t[2] for t in times) - t_first
     lat = [t[2] - t[0] for t in times]
@@ -396,6 +432,7 @@ def main():
     if not args.no_batch:
         log("mixed-size batch")
         batched = batch_report(le2d_amd, steps=args.batch_steps)
+    ref_case = reference_case_report(le2d_amd)
     tier = None
     if not args.no_tier:
         tier = tier_report(le2d_amd, steps=args.tier_steps)
@@ -422,6 +459,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         ("fp32" if main_dt == np.float64 else "fp64"): other,
+        "reference_case": ref_case,
         "batched": batched,
         "tier": tier,
         "serving": serving,

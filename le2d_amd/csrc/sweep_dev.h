@@ -32,7 +32,11 @@ __device__ __forceinline__ T superbee(T r) {
   const T r2 = T(2) * r;
   const T a = (T(1) > r2) ? r2 : T(1);
   const T b = (T(2) > r) ? r : T(2);
-  const T m = (a > b) ? a : b;
+  // (a > b) ? a : b equals the hardware max here: a and b are never NaN (a
+  // NaN r gives a = 1, b = 2), and they are zeros only together and with the
+  // same sign (a = 2r, b = r), so no signed-zero tie can pick differently.
+  const T m = __builtin_fmax(a, b);
+  // kept as a select: for r = -0 the reference returns -0
   return (T(0) > m) ? T(0) : m;
 }
 
