@@ -1,7 +1,7 @@
 // batch.hip -- many independent grids of uneven sizes and materials advanced
 // by one persistent launch per step, captured as a hipGraph.
 //
-// Every instance is a ping-pong pair of pitched planes (the single-grid
+// Every instance is a ping-pong pair of state buffers (the single-grid
 // layout) carved from one arena.  A step is one launch of k_step_batch over
 // a table of tiles (instance, strip, row segment) of all instances; each
 // tile runs the fused-step march of sweep_dev.h with its instance's pointers
@@ -278,8 +278,7 @@ int le2d_batch_commit(le2d_batch *b) {
   if (B.inst.empty()) return le2d::set_error(le2d::ERR_ARG, "batch: empty");
   size_t off = 0;
   for (auto &in : B.inst) {
-    in.pitch = le2d::plane_pitch(in.nx, B.elem);
-    in.plane = (long)(le2d::grid_bytes(B.elem, in.nx, in.ny) / 5 / B.elem);
+    le2d::grid_layout(B.elem, in.nx, in.ny, &in.pitch, &in.plane);
     const size_t bytes = le2d::grid_bytes(B.elem, in.nx, in.ny);
     for (int k = 0; k < 2; ++k) {
       in.off[k] = off;

@@ -40,8 +40,8 @@ struct Tiles {
 struct DevGrid {
   int elem = 0;        // bytes per value: 4 or 8
   int nx = 0, ny = 0;
-  long pitch = 0;      // elements between rows
-  long plane = 0;      // elements between planes
+  long pitch = 0;      // elements between rows (of one field)
+  long plane = 0;      // elements between the fields of a row
   void *buf[2] = {nullptr, nullptr};
   int cur = 0;         // buffer that holds the current state
   hipStream_t stream = nullptr;
@@ -58,7 +58,8 @@ const char *last_error();
 // Aborts with a message when no usable HIP device is present.
 int require_device();
 
-long plane_pitch(int nx, int elem);
+// Layout of one state buffer: field q of node (i, j) at q*plane + j*pitch + i.
+void grid_layout(int elem, int nx, int ny, long *pitch, long *plane);
 size_t grid_bytes(int elem, int nx, int ny);  // device bytes of one state buffer
 int grid_alloc(DevGrid *g, int elem, int nx, int ny, hipStream_t stream);
 void grid_free(DevGrid *g);

@@ -34,20 +34,26 @@ int require_device() {
   return OK;
 }
 
-// Rows start on 256-byte boundaries so every wavefront row access is aligned.
-long plane_pitch(int nx, int elem) {
+// One grid row of one field, padded to 256 bytes so every row starts aligned.
+static long field_pitch(int nx, int elem) {
   const long per = 256 / elem;
   return ((long)nx + per - 1) / per * per;
 }
 
-// Planes are separated by the pitched plane plus a 4 KiB skew, so the five
-// streams of a sweep do not start on the same HBM channel at equal offsets.
-static long plane_stride(int nx, int ny, int elem) {
-  return plane_pitch(nx, elem) * ny + 4096 / elem;
+// Row-interleaved fields: the five fields of grid row j are stored side by
+// side (vx, vy, sxx, sxy, syy), each padded to 256 bytes, and the rows follow
+// one another.  The workgroups of a sweep that work on the same rows then
+// read and write one contiguous region instead of five regions a plane
+// apart, which keeps the DRAM access stream local.
+void grid_layout(int elem, int nx, int ny, long *pitch, long *plane) {
+  (void)ny;
+  const long fp = field_pitch(nx, elem);
+  *pitch = 5 * fp;
+  *plane = fp;
 }
 
 size_t grid_bytes(int elem, int nx, int ny) {
-  return (size_t)plane_stride(nx, ny, elem) * 5 * elem;
+  return (size_t)5 * field_pitch(nx, elem) * ny * elem;
 }
 
 int grid_alloc(DevGrid *g, int elem, int nx, int ny, hipStream_t stream) {
@@ -57,8 +63,7 @@ int grid_alloc(DevGrid *g, int elem, int nx, int ny, hipStream_t stream) {
   g->elem = elem;
   g->nx = nx;
   g->ny = ny;
-  g->pitch = plane_pitch(nx, elem);
-  g->plane = plane_stride(nx, ny, elem);
+  grid_layout(elem, nx, ny, &g->pitch, &g->plane);
   g->stream = stream;
   g->cur = 0;
   (void)hipGetDevice(&g->device);
@@ -83,7 +88,7 @@ void grid_free(DevGrid *g) {
   }
 }
 
-// AOS (node-contiguous, unpitched) <-> pitched planes.  One lane per node;
+// AOS (node-contiguous, unpitched) <-> the device layout.  One lane per node;
 // the AOS side is read/written as 5 consecutive values per lane.
 template <typename T>
 __global__ void k_aos_to_planes(const T *__restrict__ aos, T *__restrict__ pl, int nx, int ny,
@@ -114,7 +119,7 @@ static int check(hipError_t e, const char *what) {
   return OK;
 }
 
-// The other ping-pong buffer stages the AOS image (it holds 5*plane >= 5*nx*ny
+// The other ping-pong buffer stages the AOS image (it holds at least 5*nx*ny
 // values and carries no state between steps).
 int grid_upload(DevGrid *g, const void *host, int soa) {
   const size_t es = g->elem;

@@ -162,8 +162,7 @@ struct Server {
 
   void init_fields(Req &r) {
     const le2d_request &q = r.spec;
-    r.pitch = plane_pitch(q.nx, q.real_size);
-    r.plane = (long)(grid_bytes(q.real_size, q.nx, q.ny) / 5 / q.real_size);
+    grid_layout(q.real_size, q.nx, q.ny, &r.pitch, &r.plane);
     const int si = q.real_size == 8 ? 1 : 0;
     char *base = arena_dev + r.off;
     (void)hipMemsetAsync(base, 0, r.bytes, stream[si]);

@@ -15,7 +15,8 @@
 // reference's order, which keeps FP64 results bit-identical when the
 // compiler does not contract multiply-adds (-ffp-contract=off).
 //
-// Layout: five planes, element (i, j) of plane q at q*plane + j*pitch + i.
+// Layout: field q (vx, vy, sxx, sxy, syy) of node (i, j) at q*plane + j*pitch + i;
+// the fields of a row lie side by side (plane = padded row, pitch = 5 rows).
 #pragma once
 
 #include "engine.h"
@@ -150,7 +151,7 @@ __device__ __forceinline__ Node<T> load_node(const T *__restrict__ p, long plane
 // Loads and stores through a wavefront-uniform row base (kept in SGPRs) and a
 // 32-bit per-lane byte offset, which maps to the SGPR-base + VGPR-offset
 // addressing of global memory instructions and keeps VGPRs free of 64-bit
-// addresses.  Planes are < 4 GiB, so the offset always fits.
+// addresses.  The offset is a column offset within one row, so it always fits.
 template <typename T>
 __device__ __forceinline__ T ld_u(const T *__restrict__ base, unsigned boff) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
