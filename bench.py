@@ -190,7 +190,8 @@ def run_batch(le2d_amd, dtype, insts, steps, plain_steps):
     b.wait()
     es = np.dtype(dtype).itemsize
     res = {"instances": len(insts), "nodes_per_step": nodes, "steps": steps,
-           "shape": {"window_lanes": fb, "segment_rows": seg},
+           "shape": {"window_lanes": fb, "segment_rows": seg,
+                     "launch": "persistent" if b.persistent() else "one workgroup per tile"},
            "batched_Gnode_per_s": round(nodes / ms_batch / 1e6, 3),
            "plain_Gnode_per_s": round(nodes / ms_plain / 1e6, 3),
            "speedup_vs_plain": round(ms_plain / ms_batch, 3),
@@ -387,6 +388,7 @@ def main():
     ap.add_argument("--no-tier", action="store_true", help="skip the two-material FP32 tier leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the second-type run")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
+    ap.add_argument("--no-ref", action="store_true", help="skip the 512x512 reference-case leg")
     ap.add_argument("--serve-window", type=float, default=1.0,
                     help="seconds of Poisson arrivals in the serving leg")
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -432,7 +434,7 @@ def main():
     if not args.no_batch:
         log("mixed-size batch")
         batched = batch_report(le2d_amd, steps=args.batch_steps)
-    ref_case = reference_case_report(le2d_amd)
+    ref_case = None if args.no_ref else reference_case_report(le2d_amd)
     tier = None
     if not args.no_tier:
         tier = tier_report(le2d_amd, steps=args.tier_steps)

@@ -122,8 +122,8 @@ int le2d_diff_norms(const le2d_view *a, const le2d_view *ref, double out[4]);
 
 /* ---------------------------------------------------------------------------
  * Batched instances: many independent grids of uneven sizes and homogeneous
- * materials, advanced together by one persistent kernel launch per step
- * (replayed from a hipGraph).  No counterpart in the reference, which steps
+ * materials, advanced together by one kernel launch per step (persistent, or
+ * one workgroup per tile; replayed from a hipGraph).  No counterpart in the reference, which steps
  * one task at a time.  Element type per batch: real_size 4 (FP32 storage and
  * arithmetic, constants formed in FP64 and rounded once) or 8 (FP64,
  * bit-identical to stepping each grid alone through le_step).  Edges clamp
@@ -153,8 +153,13 @@ int le2d_batch_read(le2d_batch *b, int inst, void *aos);
 int le2d_batch_step(le2d_batch *b, int steps);
 int le2d_batch_step_plain(le2d_batch *b, int steps);
 int le2d_batch_wait(le2d_batch *b);
-/* Launch shape: window width 64/128/256 lanes and rows per segment. */
+/* Launch shape: window width 64/128/256 lanes and rows per segment (the
+ * launch form unchanged); le2d_batch_set_launch also picks the form:
+ * persistent = 1, resident workgroups walk the tile table; 0, one workgroup
+ * per tile. */
 int le2d_batch_set_shape(le2d_batch *b, int fb, int seg);
+int le2d_batch_set_launch(le2d_batch *b, int fb, int seg, int persistent);
+int le2d_batch_get_launch(le2d_batch *b, int *persistent);
 int le2d_batch_get_shape(le2d_batch *b, int *fb, int *seg, int *ntiles, int *blocks);
 /* Mean ms per step of `steps` batched (plain = 0) or per-instance (plain = 1)
  * steps, HIP events on the batch's stream. */

@@ -80,6 +80,8 @@ _BATCH = {
     "le2d_batch_step_plain": ([_VP, ctypes.c_int], ctypes.c_int),
     "le2d_batch_wait": ([_VP], ctypes.c_int),
     "le2d_batch_set_shape": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "le2d_batch_set_launch": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "le2d_batch_get_launch": ([_VP, _IP], ctypes.c_int),
     "le2d_batch_get_shape": ([_VP, _IP, _IP, _IP, _IP], ctypes.c_int),
     "le2d_batch_time": ([_VP, ctypes.c_int, ctypes.c_int, _DP], ctypes.c_int),
     "le2d_batch_autotune": ([_VP, ctypes.c_int, _IP, _IP, _DP], ctypes.c_int),
@@ -354,8 +356,20 @@ class Batch:
     def wait(self):
         self._check(self.lib.le2d_batch_wait(self.h), "le2d_batch_wait")
 
-    def set_shape(self, fb, seg):
-        self._check(self.lib.le2d_batch_set_shape(self.h, fb, seg), "le2d_batch_set_shape")
+    def set_shape(self, fb, seg, persistent=None):
+        """Window width and rows per segment; persistent=True/False also picks
+        the launch form (None keeps it)."""
+        if persistent is None:
+            self._check(self.lib.le2d_batch_set_shape(self.h, fb, seg), "le2d_batch_set_shape")
+        else:
+            self._check(self.lib.le2d_batch_set_launch(self.h, fb, seg, int(bool(persistent))),
+                        "le2d_batch_set_launch")
+
+    def persistent(self):
+        """True when the launch form is persistent, False for one workgroup per tile."""
+        v = ctypes.c_int()
+        self._check(self.lib.le2d_batch_get_launch(self.h, ctypes.byref(v)), "get_launch")
+        return bool(v.value)
 
     def get_shape(self):
         v = [ctypes.c_int() for _ in range(4)]
@@ -414,7 +428,7 @@ class Server:
         if rc != 0:
             raise LE2DError(f"{what}: {rc} {self.lib.le2d_last_error().decode()}")
 
-    def set_shape(self, fb=256, seg=32, max_chunk=16):
+    def set_shape(self, fb=256, seg=16, max_chunk=16):
         self._check(self.lib.le2d_server_set_shape(self.h, fb, seg, max_chunk), "set_shape")
 
     def submit(self, nx, ny, steps, dtype=np.float32, dt=0.25, h=(1.0, 1.0),

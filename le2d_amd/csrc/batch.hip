@@ -1,5 +1,6 @@
 // batch.hip -- many independent grids of uneven sizes and materials advanced
-// by one persistent launch per step, captured as a hipGraph.
+// by one launch per step (persistent, or one workgroup per tile), captured as
+// a hipGraph.
 //
 // Every instance is a ping-pong pair of state buffers (the single-grid
 // layout) carved from one arena.  A step is one launch of k_step_batch over
@@ -9,8 +10,8 @@
 // stepped alone (FP64) or agrees to FP32 rounding (FP32).  All instances
 // advance together and share one buffer parity.
 //
-// The launch shape -- window width (64, 128 or 256 lanes) and rows per
-// segment -- is chosen by le2d_batch_autotune, which times candidate shapes
+// The launch shape -- window width (64, 128 or 256 lanes), rows per segment,
+// and persistent or one workgroup per tile -- is chosen by le2d_batch_autotune, which times candidate shapes
 // on the batch itself and keeps the fastest (per element size, instance
 // count and node total, for the life of the process).
 #include <algorithm>
@@ -39,8 +40,12 @@ struct Inst {
   size_t off[2];
 };
 
+// Launch shape: window width, rows per segment, and the launch form --
+// persistent (resident workgroups walk the tile table) or one workgroup per
+// tile.
 struct Shape {
   int fb, seg;
+  int persist;
 };
 
 std::mutex g_tune_mu;
@@ -67,7 +72,7 @@ struct BatchImpl {
   void *d_inst = nullptr;
   TileJob *d_tiles = nullptr;
   int ntiles = 0;
-  Shape shape{128, 32};
+  Shape shape{128, 32, 0};
   int blocks = 0;
   int parity = 0;
   hipStream_t stream = nullptr;
@@ -157,8 +162,11 @@ struct BatchImpl {
     if (rc) return rc;
     ntiles = (int)h.size();
     shape = s;
-    // persistent grid: resident workgroups on the whole device
-    if (elem == 8)
+    // persistent grid: resident workgroups on the whole device; else one
+    // workgroup per tile
+    if (!s.persist)
+      blocks = ntiles;
+    else if (elem == 8)
       blocks = s.fb == 64 ? persistent_blocks<double, 64>(ntiles)
                           : (s.fb == 128 ? persistent_blocks<double, 128>(ntiles)
                                          : persistent_blocks<double, 256>(ntiles));
@@ -172,8 +180,12 @@ struct BatchImpl {
 
   template <typename T, int FB>
   void launch_t(int par) {
-    hipLaunchKernelGGL((k_step_batch<T, FB>), dim3(blocks), dim3(FB), 0, stream,
-                       static_cast<const InstDesc<T> *>(d_inst), d_tiles, ntiles, par);
+    if (shape.persist)
+      hipLaunchKernelGGL((k_step_batch<T, FB>), dim3(blocks), dim3(FB), 0, stream,
+                         static_cast<const InstDesc<T> *>(d_inst), d_tiles, ntiles, par);
+    else if (ntiles > 0)
+      hipLaunchKernelGGL((k_step_batch_tiles<T, FB>), dim3(ntiles), dim3(FB), 0, stream,
+                         static_cast<const InstDesc<T> *>(d_inst), d_tiles, par);
   }
 
   void launch(int par) {
@@ -292,8 +304,9 @@ int le2d_batch_commit(le2d_batch *b) {
   if (rc) return rc;
   rc = B.elem == 8 ? B.upload_desc<double>() : B.upload_desc<float>();
   if (rc) return rc;
-  // default shape until autotuned: the cached winner for this batch, else 128 x 32
-  le2d::Shape s{128, 32};
+  // default shape until autotuned: the cached winner for this batch, else
+  // 128 x 32, one workgroup per tile
+  le2d::Shape s{128, 32, 0};
   {
     std::lock_guard<std::mutex> lk(le2d::g_tune_mu);
     auto it = le2d::g_tune.find({B.elem, (int)B.inst.size(), B.total_nodes() >> 20});
@@ -374,10 +387,21 @@ int le2d_batch_wait(le2d_batch *b) {
   return le2d::hcheck(hipStreamSynchronize(b->impl.stream), "batch: wait");
 }
 
-int le2d_batch_set_shape(le2d_batch *b, int fb, int seg) {
+int le2d_batch_set_launch(le2d_batch *b, int fb, int seg, int persistent) {
   if (!b || !b->impl.committed || (fb != 64 && fb != 128 && fb != 256) || seg < 1)
     return le2d::set_error(le2d::ERR_ARG, "batch: shape %d %d", fb, seg);
-  return b->impl.build_tiles({fb, seg});
+  return b->impl.build_tiles({fb, seg, persistent ? 1 : 0});
+}
+
+int le2d_batch_get_launch(le2d_batch *b, int *persistent) {
+  if (!b || !persistent) return le2d::ERR_ARG;
+  *persistent = b->impl.shape.persist;
+  return le2d::OK;
+}
+
+int le2d_batch_set_shape(le2d_batch *b, int fb, int seg) {
+  if (!b) return le2d::ERR_ARG;
+  return le2d_batch_set_launch(b, fb, seg, b->impl.shape.persist);
 }
 
 int le2d_batch_get_shape(le2d_batch *b, int *fb, int *seg, int *ntiles, int *blocks) {
@@ -422,21 +446,22 @@ int le2d_batch_autotune(le2d_batch *b, int trials, int *best_fb, int *best_seg, 
   const int segs[4] = {16, 32, 64, 128};
   le2d::Shape best{B.shape};
   double best_t = 1e30;
-  for (int fb : fbs) {
-    for (int seg : segs) {
-      int rc = B.build_tiles({fb, seg});
-      if (rc) return rc;
-      rc = B.step(2, true);  // captures the graph, warms up
-      if (rc) return rc;
-      double ms = 0;
-      rc = le2d_batch_time(b, 2 * trials, 0, &ms);
-      if (rc) return rc;
-      if (ms < best_t) {
-        best_t = ms;
-        best = {fb, seg};
+  for (int persist = 0; persist < 2; ++persist)
+    for (int fb : fbs) {
+      for (int seg : segs) {
+        int rc = B.build_tiles({fb, seg, persist});
+        if (rc) return rc;
+        rc = B.step(2, true);  // captures the graph, warms up
+        if (rc) return rc;
+        double ms = 0;
+        rc = le2d_batch_time(b, 2 * trials, 0, &ms);
+        if (rc) return rc;
+        if (ms < best_t) {
+          best_t = ms;
+          best = {fb, seg, persist};
+        }
       }
     }
-  }
   int rc = B.build_tiles(best);
   if (rc) return rc;
   {

@@ -30,7 +30,7 @@ struct TileJob {
 // is round-robin); each XCD walks its own contiguous eighth of the table, so
 // concurrently running workgroups of one XCD process neighbouring strips.
 template <typename T, int FB>
-__global__ __launch_bounds__(FB) void k_step_batch(const InstDesc<T> *__restrict__ inst,
+__global__ __launch_bounds__(FB, 3) void k_step_batch(const InstDesc<T> *__restrict__ inst,
                                                    const TileJob *__restrict__ tiles, int ntiles,
                                                    int step_parity) {
   __shared__ W4<T> sw[FB];
@@ -45,6 +45,21 @@ __global__ __launch_bounds__(FB) void k_step_batch(const InstDesc<T> *__restrict
     march_tile_alt<T, EDGE_CLAMP, FB>(d.buf[p], d.buf[p ^ 1], d.nx, d.ny, d.pitch, d.plane,
                                       tj.strip, tj.j0, tj.j1, tj.seg, d.cx, d.cy, sw, sf);
   }
+}
+
+// One workgroup per tile: the table in XCD-aware order (xcd_tile), so each
+// XCD runs a contiguous eighth of it, as the persistent form walks it.
+template <typename T, int FB>
+__global__ __launch_bounds__(FB, 3) void k_step_batch_tiles(const InstDesc<T> *__restrict__ inst,
+                                                          const TileJob *__restrict__ tiles,
+                                                          int step_parity) {
+  __shared__ W4<T> sw[FB];
+  __shared__ W4<T> sf[FB];
+  const TileJob tj = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const InstDesc<T> &d = inst[tj.inst];
+  const int p = d.par ^ step_parity;
+  march_tile_alt<T, EDGE_CLAMP, FB>(d.buf[p], d.buf[p ^ 1], d.nx, d.ny, d.pitch, d.plane,
+                                    tj.strip, tj.j0, tj.j1, tj.seg, d.cx, d.cy, sw, sf);
 }
 
 // Disk initial condition with the host le_set_ball's arithmetic: the node

@@ -16,7 +16,7 @@
 namespace le2d {
 
 template <typename T, int EDGE, int FB>
-__global__ __launch_bounds__(FB) void k_step_mat_homog(const T *__restrict__ src,
+__global__ __launch_bounds__(FB, 3) void k_step_mat_homog(const T *__restrict__ src,
                                                        T *__restrict__ dst, int nx, int ny,
                                                        long pitch, long plane,
                                                        const MatTile *__restrict__ tiles,

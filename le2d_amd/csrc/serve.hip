@@ -7,8 +7,8 @@
 // order while the budget (a device arena sub-allocated first-fit) and the
 // slot count allow, initialises their fields on the device, then launches
 // `chunk` steps in which every live request advances by one step per launch
-// -- one persistent k_step_batch launch per precision per step, FP32 and FP64
-// on their own streams so they overlap.  The chunk ends when the first live
+// -- one k_step_batch_tiles launch (one workgroup per tile) per precision per
+// step, FP32 and FP64 on their own streams so they overlap.  The chunk ends when the first live
 // request reaches its step count (or after at most `max_chunk` steps); that
 // request completes there, its memory returns to the arena and waiting
 // requests are admitted.  The arithmetic of a request does not depend on
@@ -102,7 +102,7 @@ struct Server {
   int max_live;
   int plain;
   int max_chunk = 16;
-  int fb = 256, seg = 32;
+  int fb = 256, seg = 16;
   char *arena_dev = nullptr;
   Arena arena;
   std::mutex mu;
@@ -118,7 +118,7 @@ struct Server {
   void *d_inst[2] = {nullptr, nullptr};
   TileJob *d_tiles[2] = {nullptr, nullptr};
   size_t cap_inst[2] = {0, 0}, cap_tiles[2] = {0, 0};
-  int ntiles[2] = {0, 0}, blocks[2] = {8, 8};
+  int ntiles[2] = {0, 0};
   bool dirty = true;
   // statistics
   long long completed = 0;
@@ -222,24 +222,23 @@ struct Server {
          hipStreamSynchronize(stream[si]) != hipSuccess))
       return set_error(ERR_DEVICE, "server: tables");
     ntiles[si] = (int)tiles.size();
-    blocks[si] = fb == 64 ? persistent_blocks<T, 64>(ntiles[si])
-                          : (fb == 128 ? persistent_blocks<T, 128>(ntiles[si])
-                                       : persistent_blocks<T, 256>(ntiles[si]));
     return OK;
   }
 
   template <typename T>
   void launch_batch(int si, int step_parity) {
     const InstDesc<T> *d = static_cast<const InstDesc<T> *>(d_inst[si]);
+    if (ntiles[si] <= 0) return;
+    const dim3 grid((unsigned)ntiles[si]);
     if (fb == 64)
-      hipLaunchKernelGGL((k_step_batch<T, 64>), dim3(blocks[si]), dim3(64), 0, stream[si], d,
-                         d_tiles[si], ntiles[si], step_parity);
+      hipLaunchKernelGGL((k_step_batch_tiles<T, 64>), grid, dim3(64), 0, stream[si], d,
+                         d_tiles[si], step_parity);
     else if (fb == 128)
-      hipLaunchKernelGGL((k_step_batch<T, 128>), dim3(blocks[si]), dim3(128), 0, stream[si], d,
-                         d_tiles[si], ntiles[si], step_parity);
+      hipLaunchKernelGGL((k_step_batch_tiles<T, 128>), grid, dim3(128), 0, stream[si], d,
+                         d_tiles[si], step_parity);
     else
-      hipLaunchKernelGGL((k_step_batch<T, 256>), dim3(blocks[si]), dim3(256), 0, stream[si], d,
-                         d_tiles[si], ntiles[si], step_parity);
+      hipLaunchKernelGGL((k_step_batch_tiles<T, 256>), grid, dim3(256), 0, stream[si], d,
+                         d_tiles[si], step_parity);
   }
 
   // The serial baseline's step: the single-grid kernel on the one live request.

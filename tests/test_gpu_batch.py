@@ -59,8 +59,10 @@ def test_batch_fp64_bit_identical_to_oracle():
 
 def test_batch_plain_loop_and_shapes_agree():
     res = []
-    for shape, plain in [((128, 32), False), ((64, 7), False), ((256, 1000), False),
-                         ((128, 32), True)]:
+    # (window, segment, persistent launch or one workgroup per tile)
+    for shape, plain in [((128, 32, False), False), ((64, 7, True), False),
+                         ((256, 1000, True), False), ((256, 16, False), False),
+                         ((128, 32, None), True)]:
         b, states = _make(np.float64)
         with b:
             b.set_shape(*shape)
@@ -79,7 +81,7 @@ def test_batch_autotune_keeps_results():
     with b:
         fb, seg, ms = b.autotune(1)  # advances every instance by an even count
         assert fb in (64, 128, 256) and seg in (16, 32, 64, 128) and ms > 0
-        n_tuned = 3 * 4 * 4  # 12 shapes x (2 warm-up + 2 timed) steps
+        n_tuned = 2 * 3 * 4 * 4  # 24 launch shapes x (2 warm-up + 2 timed) steps
         for k, g in enumerate(states):
             b.load(k, g)
         b.step(3)
