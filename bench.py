@@ -117,6 +117,36 @@ def reference_case_report(le2d_amd, n=512, steps=100):
                         f"{steps} steps", "fp32_vs_fp64": errs}
 
 
+def host_boundary_report(le2d_amd, n=8192, steps=100, trips=3):
+    """The boundary's host-buffer path, PCIe included: the reference driver's
+    pattern (state written into task->grid on the host, `steps` calls of
+    le_step, the state read back) and a round trip per step (upload, one
+    le_step, download)."""
+    t = le2d_amd.Task(n, n, 0.25, (1.0, 1.0), (2.0, 1.0, 1.0))
+    cx, cy = disk_centre(n)
+    t.set_ball((cx, cy), 64.0, 1.0)
+    t.step()
+    t.sync_host()  # first upload and kernel load out of the timing
+    t.lib.le_mark_host_dirty(t.ref)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t.step()
+    t.sync_host()
+    el = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    for _ in range(trips):
+        t.lib.le_mark_host_dirty(t.ref)
+        t.step()
+        t.sync_host()
+    el2 = time.perf_counter() - t1
+    t.close()
+    return {"workload": f"{n}x{n} FP64 through le_step on the host mirror",
+            "driver_pattern": {"steps": steps, "seconds": round(el, 3),
+                               "Gnode_per_s": round(n * n * steps / el / 1e9, 3)},
+            "round_trip_per_step": {"ms_per_step": round(el2 / trips * 1e3, 2),
+                                    "Gnode_per_s": round(n * n * trips / el2 / 1e9, 3)}}
+
+
 def cpu_baseline(n, budget_s=12.0):
     """Reference simple/ path (oracle/_ref, compiled from the reference's own
     sources) if present, else the oracle port; FP64, same grid and disk, a
@@ -389,6 +419,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the second-type run")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-ref", action="store_true", help="skip the 512x512 reference-case leg")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe) leg")
     ap.add_argument("--serve-window", type=float, default=1.0,
                     help="seconds of Poisson arrivals in the serving leg")
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -435,6 +466,7 @@ def main():
         log("mixed-size batch")
         batched = batch_report(le2d_amd, steps=args.batch_steps)
     ref_case = None if args.no_ref else reference_case_report(le2d_amd)
+    host = None if args.no_host else host_boundary_report(le2d_amd, n)
     tier = None
     if not args.no_tier:
         tier = tier_report(le2d_amd, steps=args.tier_steps)
@@ -462,6 +494,7 @@ def main():
         "cpu_baseline": cpu,
         ("fp32" if main_dt == np.float64 else "fp64"): other,
         "reference_case": ref_case,
+        "host_boundary": host,
         "batched": batched,
         "tier": tier,
         "serving": serving,
