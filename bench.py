@@ -85,9 +85,17 @@ def roofline(n, es, ms_step, ms_x, ms_y, dtype_name):
     # sizeof(real); one k_step_xy launch advances every node by one step.
     alg = nodes * 20 * es
     achieved = alg / (ms_step * 1e-3) / 1e9
+    traffic = measured_traffic(n, dtype_name)
+    # the fused launch moves each node's five values in and out once: half the
+    # two-sweep figure is compulsory; `traffic` is what rocprof counted
+    compulsory = nodes * 10 * es
+    actual = traffic / (ms_step * 1e-3) / 1e9 if traffic else None
     return {"bound": "hbm", "kernel": "k_step_xy", "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": measured_traffic(n, dtype_name),
+            "traffic": traffic,
+            "traffic_GBps": round(actual, 1) if actual else None,
+            "traffic_frac": round(actual / PEAK_HBM_GBS, 4) if actual else None,
+            "compulsory_bytes_per_launch": compulsory,
             "algorithmic_bytes_per_launch": alg,
             "ms_per_launch": {"k_step_xy": round(ms_step, 4), "k_sweep_x": round(ms_x, 4),
                               "k_sweep_y": round(ms_y, 4)},
