@@ -155,10 +155,10 @@ def host_boundary_report(le2d_amd, n=8192, steps=100, trips=3):
                                     "Gnode_per_s": round(n * n * trips / el2 / 1e9, 3)}}
 
 
-def cpu_baseline(n, budget_s=12.0):
+def cpu_baseline(n, budget_s=12.0, single_budget_s=6.0):
     """Reference simple/ path (oracle/_ref, compiled from the reference's own
     sources) if present, else the oracle port; FP64, same grid and disk, a
-    bounded number of steps on all host cores."""
+    bounded number of steps on all host cores, and a shorter sample on one."""
     from oracle import Oracle, Reference
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cx, cy = disk_centre(n)
@@ -184,11 +184,28 @@ def cpu_baseline(n, budget_s=12.0):
     for _ in range(steps):
         step()
     el = time.perf_counter() - t0
+    # the same path on one core: the OpenMP runtime the CPU code is linked to
+    single = None
+    try:
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        t0 = time.perf_counter()
+        k = 0
+        while k < 20 and (k < 1 or time.perf_counter() - t0 < single_budget_s):
+            step()
+            k += 1
+        el1 = time.perf_counter() - t0
+        gomp.omp_set_num_threads(cores)
+        single = {"value": n * n * k / el1 / 1e9, "cores": 1,
+                  "sample": f"{n}x{n} FP64, {k} steps, {el1:.1f} s wall, one thread"}
+    except OSError:
+        pass
     done()
     return {"value": n * n * steps / el / 1e9, "unit": "Gnode-updates/s", "cores": cores,
             "kind": kind,
             "sample": f"{n}x{n} FP64, {steps} steps after 3 warm-up steps, "
-                      f"{el:.1f} s wall, OMP_NUM_THREADS={cores}"}
+                      f"{el:.1f} s wall, OMP_NUM_THREADS={cores}",
+            "single_core": single}
 
 
 # ---------------------------------------------------------------------------
