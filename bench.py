@@ -54,9 +54,10 @@ def plain_task(le2d_amd, n, dtype):
 
 
 def run_plain(le2d_amd, n, dtype, steps, warmup, profile_steps):
-    """Times `steps` steps; then times the fused kernel and the two sweep
-    kernels per launch with HIP events.  Every run of this function advances
-    the task by warmup + steps + 2*profile_steps steps."""
+    """Times `steps` steps through le_step_n (two steps per pass); then times
+    the two-step kernel, the single-step kernel and the two sweep kernels per
+    launch with HIP events.  Every run of this function advances the task by
+    warmup + steps + 4*profile_steps steps."""
     t = plain_task(le2d_amd, n, dtype)
     t.step_n(max(1, warmup))
     t.wait()
@@ -64,40 +65,47 @@ def run_plain(le2d_amd, n, dtype, steps, warmup, profile_steps):
     t.step_n(steps)
     t.wait()
     el = time.perf_counter() - t0
+    ms_pair = t.time_pairs(profile_steps) if t.get_step_pairs() > 0 else None
     ms_step = t.time_steps(profile_steps)
     ms_x, ms_y = t.time_sweeps(profile_steps)
-    return t, el, ms_step, ms_x, ms_y
+    return t, el, ms_pair, ms_step, ms_x, ms_y
 
 
-def measured_traffic(n, dtype):
-    """HBM bytes per k_step_xy launch from the committed rocprofv3 counter
-    summary for this workload (profiles/traffic_k_step_xy_<n>_<dtype>.json),
+def measured_traffic(n, dtype, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary for this workload (profiles/traffic_<kernel>_<n>_<dtype>.json),
     or None when no such measurement exists."""
-    p = os.path.join(ROOT, "profiles", f"traffic_k_step_xy_{n}_{dtype}.json")
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{n}_{dtype}.json")
     if not os.path.exists(p):
         return None
     return json.load(open(p)).get("hbm_bytes_per_launch")
 
 
-def roofline(n, es, ms_step, ms_x, ms_y, dtype_name):
+def roofline(n, es, ms_pair, ms_step, ms_x, ms_y, dtype_name):
+    """Roofline of the launch that carries le_step_n: the two-step kernel
+    (two node-updates per node per launch) when it is on, else the
+    single-step kernel."""
     nodes = n * n
+    kernel, ms, steps = ("k_step2_xy", ms_pair, 2) if ms_pair else ("k_step_xy", ms_step, 1)
     # Algorithmic bytes per node-update: 2 sweeps x (5 reads + 5 writes) x
-    # sizeof(real); one k_step_xy launch advances every node by one step.
-    alg = nodes * 20 * es
-    achieved = alg / (ms_step * 1e-3) / 1e9
-    traffic = measured_traffic(n, dtype_name)
-    # the fused launch moves each node's five values in and out once: half the
-    # two-sweep figure is compulsory; `traffic` is what rocprof counted
+    # sizeof(real), times the node-updates per node one launch performs.
+    alg = nodes * 20 * es * steps
+    achieved = alg / (ms * 1e-3) / 1e9
+    traffic = measured_traffic(n, dtype_name, kernel)
+    # a fused launch moves each node's five values in and out once, however
+    # many steps it takes: that is its compulsory traffic; `traffic` is what
+    # rocprof counted
     compulsory = nodes * 10 * es
-    actual = traffic / (ms_step * 1e-3) / 1e9 if traffic else None
-    return {"bound": "hbm", "kernel": "k_step_xy", "achieved": round(achieved, 1),
+    actual = traffic / (ms * 1e-3) / 1e9 if traffic else None
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": traffic,
             "traffic_GBps": round(actual, 1) if actual else None,
             "traffic_frac": round(actual / PEAK_HBM_GBS, 4) if actual else None,
             "compulsory_bytes_per_launch": compulsory,
-            "algorithmic_bytes_per_launch": alg,
-            "ms_per_launch": {"k_step_xy": round(ms_step, 4), "k_sweep_x": round(ms_x, 4),
+            "algorithmic_bytes_per_launch": alg, "steps_per_launch": steps,
+            "ms_per_launch": {"k_step2_xy": round(ms_pair, 4) if ms_pair else None,
+                              "k_step_xy": round(ms_step, 4), "k_sweep_x": round(ms_x, 4),
                               "k_sweep_y": round(ms_y, 4)},
             "two_kernel_GBps": {"k_sweep_x": round(nodes * 10 * es / ms_x / 1e6, 1),
                                 "k_sweep_y": round(nodes * 10 * es / ms_y / 1e6, 1)}}
@@ -462,24 +470,24 @@ def main():
     main_dt = np.float64 if args.dtype == "f64" else np.float32
     es = np.dtype(main_dt).itemsize
     log(f"{n}x{n} {args.dtype}: {args.warmup} warm-up + {args.steps} timed steps")
-    t, el, ms_step, ms_x, ms_y = run_plain(le2d_amd, n, main_dt, args.steps, args.warmup,
-                                           args.profile_steps)
+    t, el, ms_pair, ms_step, ms_x, ms_y = run_plain(le2d_amd, n, main_dt, args.steps,
+                                                    args.warmup, args.profile_steps)
     value = n * n * args.steps / el / 1e9
-    roof = roofline(n, es, ms_step, ms_x, ms_y, args.dtype)
-    tiles = t.get_tiles()
+    roof = roofline(n, es, ms_pair, ms_step, ms_x, ms_y, args.dtype)
+    tiles = t.get_tiles() + (t.get_step_pairs(),)
     other = None
     if not args.no_fp32:
         o_dt = np.float32 if main_dt == np.float64 else np.float64
         o_name = "f32" if o_dt == np.float32 else "f64"
         log(f"{n}x{n} {o_name}")
-        t2, el2, ms2, mx2, my2 = run_plain(le2d_amd, n, o_dt, args.steps, args.warmup,
+        t2, el2, mp2, ms2, mx2, my2 = run_plain(le2d_amd, n, o_dt, args.steps, args.warmup,
                                            args.profile_steps)
         lo, hi = (t2, t) if o_dt == np.float32 else (t, t2)
         errs = rel_errors(le2d_amd.diff_norms(lo.device_view(), hi.device_view()))
         other = {"dtype": o_name, "value": round(n * n * args.steps / el2 / 1e9, 3),
                  "ms_per_step": round(el2 / args.steps * 1e3, 4),
-                 "roofline": roofline(n, np.dtype(o_dt).itemsize, ms2, mx2, my2, o_name),
-                 "fp32_vs_fp64_after_steps": args.warmup + args.steps + 2 * args.profile_steps,
+                 "roofline": roofline(n, np.dtype(o_dt).itemsize, mp2, ms2, mx2, my2, o_name),
+                 "fp32_vs_fp64_after_steps": args.warmup + args.steps + 4 * args.profile_steps,
                  "fp32_vs_fp64": errs}
         t2.close()
     t.close()
@@ -512,8 +520,8 @@ def main():
         "config": {"workload": f"plain split-step sweep, {n}x{n} grid, disk r=64 at "
                                f"({cx:.0f},{cy:.0f}), c1=2 c2=1 rho=1, dt=0.25",
                    "nx": n, "ny": n,
-                   "tiles": dict(zip(("x_rows", "y_seg", "y_block", "xy_seg", "xy_block"),
-                                     tiles))},
+                   "tiles": dict(zip(("x_rows", "y_seg", "y_block", "xy_seg", "xy_block",
+                                      "pair_seg"), tiles))},
         "hbm_copy_GBps": round(copy_gbs, 1),
         "roofline": roof,
         "cpu_baseline": cpu,

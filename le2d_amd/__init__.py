@@ -32,6 +32,9 @@ _EXT = {
     "le2d_time_sweeps": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "le2d_time_steps": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "le2d_set_step_pairs": (["T", ctypes.c_int], ctypes.c_int),
+    "le2d_get_step_pairs": (["T", ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "le2d_time_pairs": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "le2d_copy_probe": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                         ctypes.c_int),
     "le2d_device_bytes": (["T"], ctypes.c_size_t),
@@ -252,6 +255,21 @@ class Task:
     def time_steps(self, steps):
         a = ctypes.c_double()
         self._check(self.lib.le2d_time_steps(self.ref, steps, ctypes.byref(a)), "time_steps")
+        return a.value
+
+    def set_step_pairs(self, seg):
+        """Rows per segment of the two-step pass used by step_n (0: off)."""
+        self._check(self.lib.le2d_set_step_pairs(self.ref, seg), "set_step_pairs")
+
+    def get_step_pairs(self):
+        v = ctypes.c_int()
+        self._check(self.lib.le2d_get_step_pairs(self.ref, ctypes.byref(v)), "get_step_pairs")
+        return v.value
+
+    def time_pairs(self, passes):
+        """Mean ms of one two-step pass (HIP events)."""
+        a = ctypes.c_double()
+        self._check(self.lib.le2d_time_pairs(self.ref, passes, ctypes.byref(a)), "time_pairs")
         return a.value
 
     def save(self, path):

@@ -35,6 +35,7 @@ struct Tiles {
   int y_block;  // columns per y-sweep workgroup (64, 128 or 256)
   int xy_seg;   // rows per segment of the fused x+y step
   int xy_block; // lanes per fused-step workgroup (64, 128, 256); writes xy_block-4 columns
+  int pair_seg; // rows per segment of the two-step pass (0: one step per pass)
 };
 
 struct DevGrid {
@@ -46,7 +47,7 @@ struct DevGrid {
   int cur = 0;         // buffer that holds the current state
   hipStream_t stream = nullptr;
   int device = 0;
-  Tiles tiles{4, 128, 256, 64, 128};
+  Tiles tiles{4, 128, 256, 64, 128, 0};
 };
 
 // Error reporting: every function returns 0 on success or a nonzero code and
@@ -77,6 +78,10 @@ int sweep_y(DevGrid *g, const SweepCoef<T> &c, int edge);
 // One full step (x-sweep then y-sweep) in a single pass.
 template <typename T>
 int step_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy, int edge);
+// Two full steps in a single pass (clamped edges; tiles.pair_seg rows per
+// segment, which must be > 0).
+template <typename T>
+int step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy);
 
 // Streaming copy of `bytes` between two device buffers, `iters` times; mean
 // milliseconds per copy (HIP events).  The reference point for achievable

@@ -132,7 +132,16 @@ int run(le_task *t, int what, int edge, int steps) {
   if (s->mat.active && what != (DO_X | DO_Y))
     return le2d::set_error(LE2D_ERR_STATE,
                            "single sweeps are not available with a material plane; use le_step");
-  for (int k = 0; k < steps; ++k) {
+  int k = 0;
+  if (what == (DO_X | DO_Y) && !s->mat.active && edge == le2d::EDGE_CLAMP &&
+      s->g.tiles.pair_seg > 0) {
+    // two steps per pass while at least two remain
+    for (; k + 1 < steps; k += 2) {
+      rc = le2d::step2_xy<real>(&s->g, cx, cy);
+      if (rc) return rc;
+    }
+  }
+  for (; k < steps; ++k) {
     if (what == (DO_X | DO_Y)) {
       // one pass per step
       rc = s->mat.active ? le2d::step_xy_mat<real>(&s->g, &s->mat, edge)
@@ -452,6 +461,50 @@ int le2d_time_steps(le_task *task, int steps, double *ms_step) {
   s->dev_newer = true;
   if (rc) return rc;
   *ms_step = tot / steps;
+  return LE2D_OK;
+}
+
+int le2d_set_step_pairs(le_task *task, int seg) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  if (seg < 0) return le2d::set_error(LE2D_ERR_ARG, "step pairs: segment %d", seg);
+  s->g.tiles.pair_seg = seg;
+  return LE2D_OK;
+}
+
+int le2d_get_step_pairs(le_task *task, int *seg) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  *seg = s->g.tiles.pair_seg;
+  return LE2D_OK;
+}
+
+int le2d_time_pairs(le_task *task, int passes, double *ms_pass) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  if (passes < 1 || s->g.tiles.pair_seg < 1 || task->stype != ST_AOS)
+    return le2d::set_error(LE2D_ERR_ARG, "time pairs: %d passes, segment %d", passes,
+                           s->g.tiles.pair_seg);
+  const SweepCoef<real> cx = coef(task, task->h.x), cy = coef(task, task->h.y);
+  hipEvent_t ev[2];
+  for (auto &e : ev) (void)hipEventCreate(&e);
+  double tot = 0;
+  for (int k = 0; k < passes && !rc; ++k) {
+    (void)hipEventRecord(ev[0], s->stream);
+    rc = le2d::step2_xy<real>(&s->g, cx, cy);
+    (void)hipEventRecord(ev[1], s->stream);
+    (void)hipEventSynchronize(ev[1]);
+    float a = 0;
+    (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+    tot += a;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  s->dev_newer = true;
+  if (rc) return rc;
+  *ms_pass = tot / passes;
   return LE2D_OK;
 }
 

@@ -111,6 +111,21 @@ __global__ __launch_bounds__(FB, 3) void k_step_xy(const T *__restrict__ src, T 
   march_tile_alt<T, EDGE, FB>(src, dst, nx, ny, pitch, plane, strip, j0, j1, segi, cx, cy, sw, sf);
 }
 
+// Two fused steps over one grid (march_pair in sweep_dev.h).  Occupancy 2:
+// the two stages hold twice the march state of k_step_xy.
+template <typename T, int FB>
+__global__ __launch_bounds__(FB, 2) void k_step2_xy(const T *__restrict__ src, T *__restrict__ dst,
+                                                 int nx, int ny, long pitch, long plane, int seg,
+                                                 int nstrips, SweepCoef<T> cx, SweepCoef<T> cy) {
+  __shared__ W4<T> sw[FB];
+  __shared__ W4<T> sf[FB];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int strip = tile % nstrips, segi = tile / nstrips;
+  const int j0 = segi * seg;
+  const int j1 = min(j0 + seg, ny);
+  march_pair<T, FB>(src, dst, nx, ny, pitch, plane, strip, j0, j1, cx, cy, sw, sf);
+}
+
 // ---------------------------------------------------------------------------
 // Launchers.
 
@@ -148,6 +163,27 @@ int step_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy, int edge
   return OK;
 }
 
+template <typename T>
+int step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy) {
+  if (g->elem != (int)sizeof(T)) return set_error(ERR_ARG, "step2_xy: element size mismatch");
+  constexpr int FB = 256;
+  const int seg = g->tiles.pair_seg;
+  if (seg < 1) return set_error(ERR_ARG, "step2_xy: no pair segment height");
+  const int nstrips = (g->nx + FB - 9) / (FB - 8);
+  const long nsegs = (g->ny + seg - 1) / seg;
+  const long nblocks = nsegs * nstrips;
+  if (nblocks > 0x7fffffffL) return set_error(ERR_ARG, "step2_xy: grid too large");
+  hipLaunchKernelGGL((k_step2_xy<T, FB>), dim3((unsigned)nblocks), dim3(FB), 0, g->stream,
+                     static_cast<const T *>(g->buf[g->cur]), static_cast<T *>(g->buf[g->cur ^ 1]),
+                     g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx, cy);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(ERR_DEVICE, "step2_xy launch: %s", hipGetErrorString(e));
+  g->cur ^= 1;
+  return OK;
+}
+
+template int step2_xy<double>(DevGrid *, const SweepCoef<double> &, const SweepCoef<double> &);
+template int step2_xy<float>(DevGrid *, const SweepCoef<float> &, const SweepCoef<float> &);
 template int step_xy<double>(DevGrid *, const SweepCoef<double> &, const SweepCoef<double> &, int);
 template int step_xy<float>(DevGrid *, const SweepCoef<float> &, const SweepCoef<float> &, int);
 

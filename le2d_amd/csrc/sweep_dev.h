@@ -338,6 +338,138 @@ __device__ __forceinline__ void march_tile_alt(const T *__restrict__ src, T *__r
 }
 
 // ---------------------------------------------------------------------------
+// Two full steps per pass.
+//
+// A workgroup of FB lanes holds columns x0-4 .. x0+FB-5 and writes the FB-8
+// columns x0 .. x0+FB-9 of rows [j0, j1) at time n+2.  Stage 1 is the fused
+// step of time n (x-sweep of each raw row, register y-march) and emits the
+// time-(n+1) rows j0-2 .. j1+1; stage 2 x-sweeps each of them as it appears
+// and runs a second y-march that emits the time-(n+2) rows.  Raw rows
+// j0-4 .. j1+5 are read once (clamped at the grid edges, as the reference
+// clamps), so a pass moves 5 reads + 5 writes per node for two steps.
+//
+// Edges.  Each x-sweep reads its neighbour lanes through indices clamped
+// into the lanes that hold grid columns, so both stages see the reference's
+// column clamp whatever lanes outside the grid hold.  Stage-2 positions above
+// row 0 or below row ny-1 take that edge row's stage-2 value (the reference's
+// row clamp of the time-(n+1) state).  The operands and operations of every
+// update are those of two single steps, so the result is bit-identical.
+
+// x-sweep of one window row with clamped neighbour lanes: the face between
+// lanes t-1 and t reads lanes nb = (t-2, t-1, t, t+1), each clamped into the
+// lanes holding grid columns.  Meaningful on lanes 2..FB-3.
+template <typename T, int FB>
+__device__ __forceinline__ Node<T> window_xsweep_cl(Node<T> u, W4<T> *sw, W4<T> *sf, int t,
+                                                    const int4 &nb, const SweepCoef<T> &c) {
+  const W4<T> w = invariants(u.vx, u.sxx, u.vy, u.sxy, c);
+  sw[t] = w;
+  __syncthreads();
+  if (t >= 2 && t <= FB - 2) sf[t] = faces(sw[nb.x], sw[nb.y], sw[nb.z], sw[nb.w], c);
+  __syncthreads();
+  if (t >= 2 && t <= FB - 3) apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, sf[t], sf[t + 1], c);
+  return u;
+}
+
+template <typename T, int FB>
+__device__ __forceinline__ void march_pair(const T *__restrict__ src, T *__restrict__ dst, int nx,
+                                           int ny, long pitch, long plane, int strip, int j0,
+                                           int j1, const SweepCoef<T> &cx,
+                                           const SweepCoef<T> &cy, W4<T> *sw, W4<T> *sf) {
+  const int t = threadIdx.x;
+  const int x0 = strip * (FB - 8);
+  const int x = x0 - 4 + t;
+  const unsigned boff = (unsigned)min(max(x, 0), nx - 1) * (unsigned)sizeof(T);
+  const unsigned woff = (unsigned)max(x, 0) * (unsigned)sizeof(T);
+  const bool writer = t >= 4 && t <= FB - 5 && x < nx;
+  // lanes of grid column 0 and nx-1 (clamped into the window)
+  const int tlo = max(0, 4 - x0), thi = min(FB - 1, nx + 3 - x0);
+  const int4 nb = make_int4(min(max(t - 2, tlo), thi), min(max(t - 1, tlo), thi),
+                            min(max(t, tlo), thi), min(max(t + 1, tlo), thi));
+#define ROW(p) load_row_node(src + (long)ypos<EDGE_CLAMP>((p), ny) * pitch, plane, boff)
+#define XS(n) window_xsweep_cl<T, FB>((n), sw, sf, t, nb, cx)
+  // Stage 1: slots A/B/C hold the x-swept time-n rows q, q+1, q+2, F0/F1 the
+  // faces, wb the invariants of row q-1, N0/N1 the raw rows in flight (as in
+  // march_tile).  Stage 2: slots A2/B2/C2 hold the x-swept time-(n+1) rows
+  // r, r+1, r+2 (r = q-2), G0/G1 its faces, vb the invariants of row r-1.
+  Node<T> A, B, C, N0, N1, A2, B2, C2, Xm2, Xm1;
+  W4<T> F0, F1, wb, G0, G1, vb;
+  int q = j0 - 2;
+  {
+    const W4<T> w2 = inv_y(XS(ROW(q - 2)), cy);
+    wb = inv_y(XS(ROW(q - 1)), cy);
+    A = XS(ROW(q));
+    B = XS(ROW(q + 1));
+    C = XS(ROW(q + 2));
+    N0 = ROW(q + 3);
+    F0 = faces(w2, wb, inv_y(A, cy), inv_y(B, cy), cy);  // face q-1 | q
+  }
+  // Stage-1 step: time-(n+1) row q (in P0), its x-sweep into X, then the next
+  // raw row x-swept into P0.
+#define S1(P0, P1, P2, FI, FO, NC, NN, X)                                    \
+  {                                                                           \
+    NN = ROW(q + 4);                                                          \
+    const W4<T> w0 = inv_y(P0, cy);                                           \
+    FO = faces(wb, w0, inv_y(P1, cy), inv_y(P2, cy), cy); /* q | q+1 */       \
+    apply(P0.vy, P0.vx, P0.syy, P0.sxx, P0.sxy, w0, FI, FO, cy);              \
+    wb.a = w0.a;                                                              \
+    wb.c = w0.c;                                                              \
+    X = XS(P0);                                                               \
+    P0 = XS(NC);                                                              \
+    ++q;                                                                      \
+  }
+  S1(A, B, C, F0, F1, N0, N1, Xm2)  // q = j0-2
+  S1(B, C, A, F1, F0, N1, N0, Xm1)  // q = j0-1
+  S1(C, A, B, F0, F1, N0, N1, A2)   // q = j0
+  S1(A, B, C, F1, F0, N1, N0, B2)   // q = j0+1
+  if (j0 + 1 >= ny) B2 = A2;        // row clamp of the time-(n+1) state
+  if (j0 - 1 < 0) Xm1 = A2;
+  if (j0 - 2 < 0) Xm2 = Xm1;
+  vb = inv_y(Xm1, cy);
+  G0 = faces(inv_y(Xm2, cy), vb, inv_y(A2, cy), inv_y(B2, cy), cy);  // face j0-1 | j0
+  int r = j0;
+  // One stage-1 step (row q = r+2 of time n+1 into Q2, clamped past the last
+  // row), then one stage-2 step writing row r of time n+2.
+#define S2(P0, P1, P2, FI, FO, NC, NN, Q0, Q1, Q2, GI, GO)                   \
+  {                                                                           \
+    NN = ROW(q + 4);                                                          \
+    const W4<T> w0 = inv_y(P0, cy);                                           \
+    FO = faces(wb, w0, inv_y(P1, cy), inv_y(P2, cy), cy);                     \
+    apply(P0.vy, P0.vx, P0.syy, P0.sxx, P0.sxy, w0, FI, FO, cy);              \
+    wb.a = w0.a;                                                              \
+    wb.c = w0.c;                                                              \
+    {                                                                         \
+      const Node<T> xn = XS(P0);                                              \
+      if (q >= ny)                                                            \
+        Q2 = Q1;                                                              \
+      else                                                                    \
+        Q2 = xn;                                                              \
+    }                                                                         \
+    P0 = XS(NC);                                                              \
+    ++q;                                                                      \
+    const W4<T> v0 = inv_y(Q0, cy);                                           \
+    GO = faces(vb, v0, inv_y(Q1, cy), inv_y(Q2, cy), cy); /* r | r+1 */       \
+    apply(Q0.vy, Q0.vx, Q0.syy, Q0.sxx, Q0.sxy, v0, GI, GO, cy);              \
+    if (writer) store_row_node(dst + (long)r * pitch, plane, woff, Q0);      \
+    if (++r >= j1) break;                                                     \
+    vb.a = v0.a;                                                              \
+    vb.c = v0.c;                                                              \
+  }
+  // stage 1 continues at phase 4 of its six-step slot cycle, stage 2 at 0
+  for (;;) {
+    S2(B, C, A, F0, F1, N0, N1, A2, B2, C2, G0, G1)
+    S2(C, A, B, F1, F0, N1, N0, B2, C2, A2, G1, G0)
+    S2(A, B, C, F0, F1, N0, N1, C2, A2, B2, G0, G1)
+    S2(B, C, A, F1, F0, N1, N0, A2, B2, C2, G1, G0)
+    S2(C, A, B, F0, F1, N0, N1, B2, C2, A2, G0, G1)
+    S2(A, B, C, F1, F0, N1, N0, C2, A2, B2, G1, G0)
+  }
+#undef S2
+#undef S1
+#undef XS
+#undef ROW
+}
+
+// ---------------------------------------------------------------------------
 // Heterogeneous media: node-local march for tiles that straddle a material
 // interface.
 //
