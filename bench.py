@@ -254,7 +254,9 @@ def run_batch(le2d_amd, dtype, insts, steps, plain_steps):
     es = np.dtype(dtype).itemsize
     res = {"instances": len(insts), "nodes_per_step": nodes, "steps": steps,
            "shape": {"window_lanes": fb, "segment_rows": seg,
-                     "launch": "persistent" if b.persistent() else "one workgroup per tile"},
+                     "launch": {"tiles": "one workgroup per tile, one step",
+                                "persistent": "persistent, one step",
+                                "pairs": "one workgroup per tile, two steps"}[b.form()]},
            "batched_Gnode_per_s": round(nodes / ms_batch / 1e6, 3),
            "plain_Gnode_per_s": round(nodes / ms_plain / 1e6, 3),
            "speedup_vs_plain": round(ms_plain / ms_batch, 3),

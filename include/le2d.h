@@ -133,8 +133,8 @@ int le2d_diff_norms(const le2d_view *a, const le2d_view *ref, double out[4]);
 
 /* ---------------------------------------------------------------------------
  * Batched instances: many independent grids of uneven sizes and homogeneous
- * materials, advanced together by one kernel launch per step (persistent, or
- * one workgroup per tile; replayed from a hipGraph).  No counterpart in the reference, which steps
+ * materials, advanced together by one kernel launch per step or per two
+ * steps (replayed from a hipGraph).  No counterpart in the reference, which steps
  * one task at a time.  Element type per batch: real_size 4 (FP32 storage and
  * arithmetic, constants formed in FP64 and rounded once) or 8 (FP64,
  * bit-identical to stepping each grid alone through le_step).  Edges clamp
@@ -160,17 +160,23 @@ int le2d_batch_set_ball(le2d_batch *b, int inst, double cx, double cy, double r,
 int le2d_batch_load(le2d_batch *b, int inst, const void *aos);
 int le2d_batch_read(le2d_batch *b, int inst, void *aos);
 /* `steps` steps of every instance: the batched launch (graph replay) or, for
- * comparison, one plain launch per instance per step. */
+ * comparison, the plain single-grid launches per instance that le_step_n
+ * makes (two steps per launch while two remain, then one). */
 int le2d_batch_step(le2d_batch *b, int steps);
 int le2d_batch_step_plain(le2d_batch *b, int steps);
 int le2d_batch_wait(le2d_batch *b);
 /* Launch shape: window width 64/128/256 lanes and rows per segment (the
  * launch form unchanged); le2d_batch_set_launch also picks the form:
- * persistent = 1, resident workgroups walk the tile table; 0, one workgroup
- * per tile. */
+ * LE2D_BATCH_TILES, one workgroup per tile; LE2D_BATCH_PERSISTENT, resident
+ * workgroups walk the tile table; LE2D_BATCH_PAIRS, one workgroup per tile
+ * and two steps per launch (window 256; an odd step count ends with one
+ * single-step launch).  All forms give the same results. */
+#define LE2D_BATCH_TILES 0
+#define LE2D_BATCH_PERSISTENT 1
+#define LE2D_BATCH_PAIRS 2
 int le2d_batch_set_shape(le2d_batch *b, int fb, int seg);
-int le2d_batch_set_launch(le2d_batch *b, int fb, int seg, int persistent);
-int le2d_batch_get_launch(le2d_batch *b, int *persistent);
+int le2d_batch_set_launch(le2d_batch *b, int fb, int seg, int form);
+int le2d_batch_get_launch(le2d_batch *b, int *form);
 int le2d_batch_get_shape(le2d_batch *b, int *fb, int *seg, int *ntiles, int *blocks);
 /* Mean ms per step of `steps` batched (plain = 0) or per-instance (plain = 1)
  * steps, HIP events on the batch's stream. */
@@ -215,6 +221,9 @@ le2d_server *le2d_server_create(size_t hbm_budget, int max_live, int plain);
 /* Launch shape of the batched steps and the longest run of steps between
  * admission points. */
 int le2d_server_set_shape(le2d_server *srv, int fb, int seg, int max_chunk);
+/* Rows per segment of the two-step launches (default 64); 0: one step per
+ * launch.  Results do not change. */
+int le2d_server_set_pairs(le2d_server *srv, int pair_seg);
 /* Queue a request; returns its id (>= 0) or a negative error code
  * (-LE2D_ERR_BUDGET when it can never fit). */
 long long le2d_server_submit(le2d_server *srv, const le2d_request *req);

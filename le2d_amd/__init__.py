@@ -93,6 +93,7 @@ _BATCH = {
                         ctypes.c_int),
     "le2d_server_create": ([ctypes.c_size_t, ctypes.c_int, ctypes.c_int], _VP),
     "le2d_server_set_shape": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "le2d_server_set_pairs": ([_VP, ctypes.c_int], ctypes.c_int),
     "le2d_server_submit": ([_VP, ctypes.POINTER(le2d_request)], ctypes.c_longlong),
     "le2d_server_wait": ([_VP, ctypes.c_longlong, ctypes.c_double], ctypes.c_int),
     "le2d_server_drain": ([_VP, ctypes.c_double], ctypes.c_int),
@@ -374,20 +375,22 @@ class Batch:
     def wait(self):
         self._check(self.lib.le2d_batch_wait(self.h), "le2d_batch_wait")
 
-    def set_shape(self, fb, seg, persistent=None):
-        """Window width and rows per segment; persistent=True/False also picks
-        the launch form (None keeps it)."""
-        if persistent is None:
+    FORMS = {"tiles": 0, "persistent": 1, "pairs": 2}
+
+    def set_shape(self, fb, seg, form=None):
+        """Window width and rows per segment; form ("tiles", "persistent" or
+        "pairs") also picks the launch form (None keeps it)."""
+        if form is None:
             self._check(self.lib.le2d_batch_set_shape(self.h, fb, seg), "le2d_batch_set_shape")
         else:
-            self._check(self.lib.le2d_batch_set_launch(self.h, fb, seg, int(bool(persistent))),
+            self._check(self.lib.le2d_batch_set_launch(self.h, fb, seg, self.FORMS[form]),
                         "le2d_batch_set_launch")
 
-    def persistent(self):
-        """True when the launch form is persistent, False for one workgroup per tile."""
+    def form(self):
+        """The launch form: "tiles", "persistent" or "pairs"."""
         v = ctypes.c_int()
         self._check(self.lib.le2d_batch_get_launch(self.h, ctypes.byref(v)), "get_launch")
-        return bool(v.value)
+        return {n: k for k, n in self.FORMS.items()}[v.value]
 
     def get_shape(self):
         v = [ctypes.c_int() for _ in range(4)]
@@ -448,6 +451,10 @@ class Server:
 
     def set_shape(self, fb=256, seg=16, max_chunk=16):
         self._check(self.lib.le2d_server_set_shape(self.h, fb, seg, max_chunk), "set_shape")
+
+    def set_pairs(self, pair_seg=64):
+        """Rows per segment of the two-step launches; 0: one step per launch."""
+        self._check(self.lib.le2d_server_set_pairs(self.h, pair_seg), "set_pairs")
 
     def submit(self, nx, ny, steps, dtype=np.float32, dt=0.25, h=(1.0, 1.0),
                mat=(2.0, 1.0, 1.0), disk=(None, None, 10.0, 1.0), keep=False):

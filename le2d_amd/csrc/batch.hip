@@ -1,6 +1,6 @@
 // batch.hip -- many independent grids of uneven sizes and materials advanced
-// by one launch per step (persistent, or one workgroup per tile), captured as
-// a hipGraph.
+// by one launch per step (persistent, or one workgroup per tile) or per two
+// steps, captured as a hipGraph.
 //
 // Every instance is a ping-pong pair of state buffers (the single-grid
 // layout) carved from one arena.  A step is one launch of k_step_batch over
@@ -11,7 +11,7 @@
 // advance together and share one buffer parity.
 //
 // The launch shape -- window width (64, 128 or 256 lanes), rows per segment,
-// and persistent or one workgroup per tile -- is chosen by le2d_batch_autotune, which times candidate shapes
+// and the launch form -- is chosen by le2d_batch_autotune, which times candidate shapes
 // on the batch itself and keeps the fastest (per element size, instance
 // count and node total, for the life of the process).
 #include <algorithm>
@@ -40,12 +40,13 @@ struct Inst {
   size_t off[2];
 };
 
-// Launch shape: window width, rows per segment, and the launch form --
-// persistent (resident workgroups walk the tile table) or one workgroup per
-// tile.
+// Launch shape: window width, rows per segment, and the launch form.
+enum { FORM_TILES = 0, FORM_PERSISTENT = 1, FORM_PAIRS = 2 };
 struct Shape {
   int fb, seg;
-  int persist;
+  int form;  // FORM_TILES: one workgroup per tile; FORM_PERSISTENT: resident
+             // workgroups walk the table; FORM_PAIRS: one workgroup per tile of
+             // the two-step table, two steps per launch (fb 256)
 };
 
 std::mutex g_tune_mu;
@@ -71,8 +72,10 @@ struct BatchImpl {
   size_t arena_bytes = 0;
   void *d_inst = nullptr;
   TileJob *d_tiles = nullptr;
+  TileJob *d_tiles1 = nullptr;  // FORM_PAIRS: single-step table for odd remainders
+  int ntiles1 = 0;
   int ntiles = 0;
-  Shape shape{128, 32, 0};
+  Shape shape{256, 64, FORM_PAIRS};
   int blocks = 0;
   int parity = 0;
   hipStream_t stream = nullptr;
@@ -94,6 +97,7 @@ struct BatchImpl {
     drop_graphs();
     if (stream) (void)hipStreamSynchronize(stream);
     if (d_tiles) (void)hipFree(d_tiles);
+    if (d_tiles1) (void)hipFree(d_tiles1);
     if (d_inst) (void)hipFree(d_inst);
     if (arena) (void)hipFree(arena);
     if (stream) (void)hipStreamDestroy(stream);
@@ -141,30 +145,40 @@ struct BatchImpl {
                   "batch: descriptors");
   }
 
-  // Tile table for a shape: instances in order, then segments, then strips,
-  // so neighbouring entries share halo columns.
-  int build_tiles(Shape s) {
+  // Tile table: instances in order, then segments, then strips, so
+  // neighbouring entries share halo columns.  `width` output columns per
+  // strip.
+  int upload_table(int width, int seg_rows, TileJob **d, int *n) {
     std::vector<TileJob> h;
     for (size_t i = 0; i < inst.size(); ++i) {
       const Inst &in = inst[i];
-      const int strips = (in.nx + s.fb - 5) / (s.fb - 4);
-      const int seg = std::max(1, std::min(s.seg, in.ny));
+      const int strips = (in.nx + width - 1) / width;
+      const int seg = std::max(1, std::min(seg_rows, in.ny));
       for (int j0 = 0; j0 < in.ny; j0 += seg)
         for (int st = 0; st < strips; ++st)
           h.push_back({(int)i, st, j0, std::min(j0 + seg, in.ny), j0 / seg});
     }
-    if (d_tiles) (void)hipFree(d_tiles);
-    d_tiles = nullptr;
-    if (hipMalloc(&d_tiles, std::max<size_t>(1, h.size()) * sizeof(TileJob)) != hipSuccess)
+    if (*d) (void)hipFree(*d);
+    *d = nullptr;
+    if (hipMalloc(d, std::max<size_t>(1, h.size()) * sizeof(TileJob)) != hipSuccess)
       return set_error(ERR_NOMEM, "batch: tile table");
-    int rc = hcheck(hipMemcpy(d_tiles, h.data(), h.size() * sizeof(TileJob), hipMemcpyHostToDevice),
-                    "batch: tiles");
+    *n = (int)h.size();
+    return hcheck(hipMemcpy(*d, h.data(), h.size() * sizeof(TileJob), hipMemcpyHostToDevice),
+                  "batch: tiles");
+  }
+
+  int build_tiles(Shape s) {
+    if (s.form == FORM_PAIRS) s.fb = 256;
+    int rc = upload_table(s.form == FORM_PAIRS ? s.fb - 8 : s.fb - 4, s.seg, &d_tiles, &ntiles);
     if (rc) return rc;
-    ntiles = (int)h.size();
+    if (s.form == FORM_PAIRS) {
+      rc = upload_table(s.fb - 4, s.seg, &d_tiles1, &ntiles1);
+      if (rc) return rc;
+    }
     shape = s;
     // persistent grid: resident workgroups on the whole device; else one
     // workgroup per tile
-    if (!s.persist)
+    if (s.form != FORM_PERSISTENT)
       blocks = ntiles;
     else if (elem == 8)
       blocks = s.fb == 64 ? persistent_blocks<double, 64>(ntiles)
@@ -180,12 +194,28 @@ struct BatchImpl {
 
   template <typename T, int FB>
   void launch_t(int par) {
-    if (shape.persist)
-      hipLaunchKernelGGL((k_step_batch<T, FB>), dim3(blocks), dim3(FB), 0, stream,
-                         static_cast<const InstDesc<T> *>(d_inst), d_tiles, ntiles, par);
-    else if (ntiles > 0)
-      hipLaunchKernelGGL((k_step_batch_tiles<T, FB>), dim3(ntiles), dim3(FB), 0, stream,
-                         static_cast<const InstDesc<T> *>(d_inst), d_tiles, par);
+    if (ntiles <= 0) return;
+    const InstDesc<T> *d = static_cast<const InstDesc<T> *>(d_inst);
+    if (shape.form == FORM_PERSISTENT)
+      hipLaunchKernelGGL((k_step_batch<T, FB>), dim3(blocks), dim3(FB), 0, stream, d, d_tiles,
+                         ntiles, par);
+    else if (shape.form == FORM_PAIRS)
+      hipLaunchKernelGGL((k_step_batch_pairs<T, 256>), dim3(ntiles), dim3(256), 0, stream, d,
+                         d_tiles, par);
+    else
+      hipLaunchKernelGGL((k_step_batch_tiles<T, FB>), dim3(ntiles), dim3(FB), 0, stream, d,
+                         d_tiles, par);
+  }
+
+  // FORM_PAIRS: one single step (the odd remainder of a step count).
+  void launch_single(int par) {
+    if (ntiles1 <= 0) return;
+    if (elem == 8)
+      hipLaunchKernelGGL((k_step_batch_tiles<double, 256>), dim3(ntiles1), dim3(256), 0, stream,
+                         static_cast<const InstDesc<double> *>(d_inst), d_tiles1, par);
+    else
+      hipLaunchKernelGGL((k_step_batch_tiles<float, 256>), dim3(ntiles1), dim3(256), 0, stream,
+                         static_cast<const InstDesc<float> *>(d_inst), d_tiles1, par);
   }
 
   void launch(int par) {
@@ -214,17 +244,22 @@ struct BatchImpl {
 
   int step(int steps, bool use_graph) {
     if (!committed) return set_error(ERR_STATE, "batch: not committed");
+    const int spl = shape.form == FORM_PAIRS ? 2 : 1;  // steps per launch
     int k = 0;
-    if (use_graph && steps >= 2) {
-      int rc = ensure_graph(parity);
+    if (use_graph && steps >= 2 * spl) {
+      int rc = ensure_graph(parity);  // two launches: the parity comes back
       if (rc) return rc;
-      for (; k + 1 < steps; k += 2) {
+      for (; k + 2 * spl <= steps; k += 2 * spl) {
         rc = hcheck(hipGraphLaunch(exec[parity], stream), "graph launch");
         if (rc) return rc;
       }
     }
-    for (; k < steps; ++k) {
+    for (; k + spl <= steps; k += spl) {
       launch(parity);
+      parity ^= 1;
+    }
+    for (; k < steps; ++k) {  // odd remainder of two-step launches
+      launch_single(parity);
       parity ^= 1;
     }
     steps_done += steps;
@@ -305,8 +340,8 @@ int le2d_batch_commit(le2d_batch *b) {
   rc = B.elem == 8 ? B.upload_desc<double>() : B.upload_desc<float>();
   if (rc) return rc;
   // default shape until autotuned: the cached winner for this batch, else
-  // 128 x 32, one workgroup per tile
-  le2d::Shape s{128, 32, 0};
+  // the two-step form with 64-row segments
+  le2d::Shape s{256, 64, le2d::FORM_PAIRS};
   {
     std::lock_guard<std::mutex> lk(le2d::g_tune_mu);
     auto it = le2d::g_tune.find({B.elem, (int)B.inst.size(), B.total_nodes() >> 20});
@@ -361,22 +396,28 @@ int le2d_batch_step_plain(le2d_batch *b, int steps) {
   if (!b || !b->impl.committed || steps < 0)
     return le2d::set_error(le2d::ERR_ARG, "batch: plain step");
   BatchImpl &B = b->impl;
-  for (int k = 0; k < steps; ++k) {
+  // what stepping each grid alone through le_step_n does: two steps per
+  // launch while two remain (its default segment height), then one
+  for (int k = 0; k < steps;) {
+    const bool pair = k + 1 < steps;
     for (size_t i = 0; i < B.inst.size(); ++i) {
       le2d::DevGrid g = B.view((int)i);
       int rc;
       if (B.elem == 8) {
         le2d::SweepCoef<double> cx, cy;
         le2d::inst_coef<double>(B.inst[i], &cx, &cy);
-        rc = le2d::step_xy<double>(&g, cx, cy, le2d::EDGE_CLAMP);
+        rc = pair ? le2d::step2_xy<double>(&g, cx, cy)
+                  : le2d::step_xy<double>(&g, cx, cy, le2d::EDGE_CLAMP);
       } else {
         le2d::SweepCoef<float> cx, cy;
         le2d::inst_coef<float>(B.inst[i], &cx, &cy);
-        rc = le2d::step_xy<float>(&g, cx, cy, le2d::EDGE_CLAMP);
+        rc = pair ? le2d::step2_xy<float>(&g, cx, cy)
+                  : le2d::step_xy<float>(&g, cx, cy, le2d::EDGE_CLAMP);
       }
       if (rc) return rc;
     }
     B.parity ^= 1;
+    k += pair ? 2 : 1;
   }
   B.steps_done += steps;
   return le2d::OK;
@@ -387,21 +428,25 @@ int le2d_batch_wait(le2d_batch *b) {
   return le2d::hcheck(hipStreamSynchronize(b->impl.stream), "batch: wait");
 }
 
-int le2d_batch_set_launch(le2d_batch *b, int fb, int seg, int persistent) {
-  if (!b || !b->impl.committed || (fb != 64 && fb != 128 && fb != 256) || seg < 1)
-    return le2d::set_error(le2d::ERR_ARG, "batch: shape %d %d", fb, seg);
-  return b->impl.build_tiles({fb, seg, persistent ? 1 : 0});
+int le2d_batch_set_launch(le2d_batch *b, int fb, int seg, int form) {
+  if (!b || !b->impl.committed || (fb != 64 && fb != 128 && fb != 256) || seg < 1 ||
+      form < le2d::FORM_TILES || form > le2d::FORM_PAIRS ||
+      (form == le2d::FORM_PAIRS && fb != 256))
+    return le2d::set_error(le2d::ERR_ARG, "batch: shape %d %d form %d", fb, seg, form);
+  return b->impl.build_tiles({fb, seg, form});
 }
 
-int le2d_batch_get_launch(le2d_batch *b, int *persistent) {
-  if (!b || !persistent) return le2d::ERR_ARG;
-  *persistent = b->impl.shape.persist;
+int le2d_batch_get_launch(le2d_batch *b, int *form) {
+  if (!b || !form) return le2d::ERR_ARG;
+  *form = b->impl.shape.form;
   return le2d::OK;
 }
 
 int le2d_batch_set_shape(le2d_batch *b, int fb, int seg) {
   if (!b) return le2d::ERR_ARG;
-  return le2d_batch_set_launch(b, fb, seg, b->impl.shape.persist);
+  const int form = b->impl.shape.form == le2d::FORM_PAIRS && fb != 256 ? le2d::FORM_TILES
+                                                                        : b->impl.shape.form;
+  return le2d_batch_set_launch(b, fb, seg, form);
 }
 
 int le2d_batch_get_shape(le2d_batch *b, int *fb, int *seg, int *ntiles, int *blocks) {
@@ -435,33 +480,37 @@ int le2d_batch_time(le2d_batch *b, int steps, int plain, double *ms) {
   return le2d::OK;
 }
 
-// Times every candidate shape for `trials` steps (after one warm-up step) and
-// keeps the fastest.  Stepping the batch is part of tuning: the instances
-// advance by (1 + trials) steps per candidate, an even number in total, so
-// callers tune before loading their initial state or account for it.
+// Times every candidate shape and keeps the fastest (per step): the
+// single-step forms at every window width and segment height, and the
+// two-step form at every segment height.  Stepping the batch is part of
+// tuning: the instances advance by an even number of steps per candidate,
+// so callers tune before loading their initial state or account for it.
 int le2d_batch_autotune(le2d_batch *b, int trials, int *best_fb, int *best_seg, double *best_ms) {
   if (!b || !b->impl.committed || trials < 1) return le2d::ERR_ARG;
   BatchImpl &B = b->impl;
   const int fbs[3] = {64, 128, 256};
   const int segs[4] = {16, 32, 64, 128};
+  std::vector<le2d::Shape> cand;
+  for (int form : {le2d::FORM_TILES, le2d::FORM_PERSISTENT})
+    for (int fb : fbs)
+      for (int seg : segs) cand.push_back({fb, seg, form});
+  for (int seg : segs) cand.push_back({256, seg, le2d::FORM_PAIRS});
   le2d::Shape best{B.shape};
   double best_t = 1e30;
-  for (int persist = 0; persist < 2; ++persist)
-    for (int fb : fbs) {
-      for (int seg : segs) {
-        int rc = B.build_tiles({fb, seg, persist});
-        if (rc) return rc;
-        rc = B.step(2, true);  // captures the graph, warms up
-        if (rc) return rc;
-        double ms = 0;
-        rc = le2d_batch_time(b, 2 * trials, 0, &ms);
-        if (rc) return rc;
-        if (ms < best_t) {
-          best_t = ms;
-          best = {fb, seg, persist};
-        }
-      }
+  for (const le2d::Shape &c : cand) {
+    int rc = B.build_tiles(c);
+    if (rc) return rc;
+    const int spl = c.form == le2d::FORM_PAIRS ? 2 : 1;
+    rc = B.step(2 * spl, true);  // captures the graph, warms up
+    if (rc) return rc;
+    double ms = 0;
+    rc = le2d_batch_time(b, 2 * spl * trials, 0, &ms);
+    if (rc) return rc;
+    if (ms < best_t) {
+      best_t = ms;
+      best = c;
     }
+  }
   int rc = B.build_tiles(best);
   if (rc) return rc;
   {

@@ -62,6 +62,21 @@ __global__ __launch_bounds__(FB, 3) void k_step_batch_tiles(const InstDesc<T> *_
                                     tj.strip, tj.j0, tj.j1, tj.seg, d.cx, d.cy, sw, sf);
 }
 
+// One workgroup per tile of the two-step table (strips of 248 columns): each
+// launch advances every instance by two steps (march_pair).
+template <typename T, int FB>
+__global__ __launch_bounds__(FB, 2) void k_step_batch_pairs(const InstDesc<T> *__restrict__ inst,
+                                                          const TileJob *__restrict__ tiles,
+                                                          int step_parity) {
+  __shared__ W4<T> sw[FB];
+  __shared__ W4<T> sf[FB];
+  const TileJob tj = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const InstDesc<T> &d = inst[tj.inst];
+  const int p = d.par ^ step_parity;
+  march_pair<T, FB>(d.buf[p], d.buf[p ^ 1], d.nx, d.ny, d.pitch, d.plane, tj.strip, tj.j0, tj.j1,
+                    d.cx, d.cy, sw, sf);
+}
+
 // Disk initial condition with the host le_set_ball's arithmetic: the node
 // position and offset in T, the distance as the double square root of the
 // T-rounded sum of squares, strict comparison with r (simple/le_core.c:71-94).

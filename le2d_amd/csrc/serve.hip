@@ -7,7 +7,8 @@
 // order while the budget (a device arena sub-allocated first-fit) and the
 // slot count allow, initialises their fields on the device, then launches
 // `chunk` steps in which every live request advances by one step per launch
-// -- one k_step_batch_tiles launch (one workgroup per tile) per precision per
+// -- one k_step_batch_pairs launch (one workgroup per tile, two steps) per
+// precision per two steps, and a k_step_batch_tiles launch for an odd last
 // step, FP32 and FP64 on their own streams so they overlap.  The chunk ends when the first live
 // request reaches its step count (or after at most `max_chunk` steps); that
 // request completes there, its memory returns to the arena and waiting
@@ -103,6 +104,7 @@ struct Server {
   int plain;
   int max_chunk = 16;
   int fb = 256, seg = 16;
+  int pair_seg = 64;  // two-step launches (segment rows); 0: one step per launch
   char *arena_dev = nullptr;
   Arena arena;
   std::mutex mu;
@@ -117,8 +119,9 @@ struct Server {
   hipStream_t stream[2] = {nullptr, nullptr};  // [0] FP32, [1] FP64
   void *d_inst[2] = {nullptr, nullptr};
   TileJob *d_tiles[2] = {nullptr, nullptr};
-  size_t cap_inst[2] = {0, 0}, cap_tiles[2] = {0, 0};
-  int ntiles[2] = {0, 0};
+  TileJob *d_tiles2[2] = {nullptr, nullptr};  // two-step table
+  size_t cap_inst[2] = {0, 0}, cap_tiles[2] = {0, 0}, cap_tiles2[2] = {0, 0};
+  int ntiles[2] = {0, 0}, ntiles2[2] = {0, 0};
   bool dirty = true;
   // statistics
   long long completed = 0;
@@ -179,10 +182,23 @@ struct Server {
 
   char *buf(const Req &r, int k) { return arena_dev + r.off + k * (r.bytes / 2); }
 
+  // Device copy of a tile table, grown as needed (in stream order).
+  int put_tiles(int si, const std::vector<TileJob> &tiles, TileJob **d, size_t *cap) {
+    if (tiles.size() * sizeof(TileJob) > *cap) {
+      if (*d) (void)hipFree(*d);
+      *cap = std::max<size_t>(tiles.size() * sizeof(TileJob) * 2, 65536);
+      if (hipMalloc((void **)d, *cap) != hipSuccess) return set_error(ERR_NOMEM, "server: tiles");
+    }
+    if (!tiles.empty() && hipMemcpyAsync(*d, tiles.data(), tiles.size() * sizeof(TileJob),
+                                         hipMemcpyHostToDevice, stream[si]) != hipSuccess)
+      return set_error(ERR_DEVICE, "server: tiles");
+    return OK;
+  }
+
   template <typename T>
   int build_tables(int si, const std::vector<Req *> &set) {
     std::vector<InstDesc<T>> desc(set.size());
-    std::vector<TileJob> tiles;
+    std::vector<TileJob> tiles, tiles2;
     for (size_t i = 0; i < set.size(); ++i) {
       const Req &r = *set[i];
       const le2d_request &q = r.spec;
@@ -200,6 +216,13 @@ struct Server {
       for (int j0 = 0; j0 < q.ny; j0 += sg)
         for (int s = 0; s < strips; ++s)
           tiles.push_back({(int)i, s, j0, std::min(j0 + sg, q.ny), j0 / sg});
+      if (pair_seg > 0) {  // strips of 248 output columns for k_step_batch_pairs<T, 256>
+        const int strips2 = (q.nx + 247) / 248;
+        const int sg2 = std::max(1, std::min(pair_seg, q.ny));
+        for (int j0 = 0; j0 < q.ny; j0 += sg2)
+          for (int s = 0; s < strips2; ++s)
+            tiles2.push_back({(int)i, s, j0, std::min(j0 + sg2, q.ny), j0 / sg2});
+      }
     }
     if (desc.size() * sizeof(InstDesc<T>) > cap_inst[si]) {
       if (d_inst[si]) (void)hipFree(d_inst[si]);
@@ -207,21 +230,17 @@ struct Server {
       if (hipMalloc(&d_inst[si], cap_inst[si]) != hipSuccess)
         return set_error(ERR_NOMEM, "server: descriptors");
     }
-    if (tiles.size() * sizeof(TileJob) > cap_tiles[si]) {
-      if (d_tiles[si]) (void)hipFree(d_tiles[si]);
-      cap_tiles[si] = std::max<size_t>(tiles.size() * sizeof(TileJob) * 2, 65536);
-      if (hipMalloc((void **)&d_tiles[si], cap_tiles[si]) != hipSuccess)
-        return set_error(ERR_NOMEM, "server: tiles");
-    }
     // tables are read by launches queued on this stream: copy in stream order
-    if (!desc.empty() &&
-        (hipMemcpyAsync(d_inst[si], desc.data(), desc.size() * sizeof(InstDesc<T>),
-                        hipMemcpyHostToDevice, stream[si]) != hipSuccess ||
-         hipMemcpyAsync(d_tiles[si], tiles.data(), tiles.size() * sizeof(TileJob),
-                        hipMemcpyHostToDevice, stream[si]) != hipSuccess ||
-         hipStreamSynchronize(stream[si]) != hipSuccess))
+    if (!desc.empty() && hipMemcpyAsync(d_inst[si], desc.data(), desc.size() * sizeof(InstDesc<T>),
+                                        hipMemcpyHostToDevice, stream[si]) != hipSuccess)
       return set_error(ERR_DEVICE, "server: tables");
+    int rc = put_tiles(si, tiles, &d_tiles[si], &cap_tiles[si]);
+    if (!rc) rc = put_tiles(si, tiles2, &d_tiles2[si], &cap_tiles2[si]);
+    if (!rc && hipStreamSynchronize(stream[si]) != hipSuccess)
+      rc = set_error(ERR_DEVICE, "server: tables");
+    if (rc) return rc;
     ntiles[si] = (int)tiles.size();
+    ntiles2[si] = (int)tiles2.size();
     return OK;
   }
 
@@ -239,6 +258,15 @@ struct Server {
     else
       hipLaunchKernelGGL((k_step_batch_tiles<T, 256>), grid, dim3(256), 0, stream[si], d,
                          d_tiles[si], step_parity);
+  }
+
+  // Two steps of every live request of one precision.
+  template <typename T>
+  void launch_pairs(int si, int step_parity) {
+    if (ntiles2[si] <= 0) return;
+    hipLaunchKernelGGL((k_step_batch_pairs<T, 256>), dim3((unsigned)ntiles2[si]), dim3(256), 0,
+                       stream[si], static_cast<const InstDesc<T> *>(d_inst[si]), d_tiles2[si],
+                       step_parity);
   }
 
   // The serial baseline's step: the single-grid kernel on the one live request.
@@ -293,6 +321,7 @@ struct Server {
       }
       const double t0 = now_s();
       int rc = OK;
+      int launches = 0;
       if (plain) {
         Req &r = set[0].empty() ? *set[1][0] : *set[0][0];
         for (int s = 0; s < chunk && !rc; ++s) {
@@ -305,9 +334,17 @@ struct Server {
           if (!rc && !set[1].empty()) rc = build_tables<double>(1, set[1]);
           dirty = false;
         }
-        for (int s = 0; s < chunk && !rc; ++s) {
-          if (!set[0].empty()) launch_batch<float>(0, s & 1);
-          if (!set[1].empty()) launch_batch<double>(1, s & 1);
+        // two steps per launch while two remain, then one; each launch flips
+        // the buffer parity
+        int s = 0;
+        if (pair_seg > 0)
+          for (; s + 2 <= chunk && !rc; s += 2, ++launches) {
+            if (!set[0].empty()) launch_pairs<float>(0, launches & 1);
+            if (!set[1].empty()) launch_pairs<double>(1, launches & 1);
+          }
+        for (; s < chunk && !rc; ++s, ++launches) {
+          if (!set[0].empty()) launch_batch<float>(0, launches & 1);
+          if (!set[1].empty()) launch_batch<double>(1, launches & 1);
         }
         if (!rc && hipGetLastError() != hipSuccess) rc = set_error(ERR_DEVICE, "server: launch");
       }
@@ -323,7 +360,7 @@ struct Server {
       std::vector<long long> still;
       for (long long id : live) {
         Req &r = reqs[id];
-        if (!plain) r.cur ^= chunk & 1;
+        if (!plain) r.cur ^= launches & 1;
         r.left -= chunk;
         node_updates += (double)r.spec.nx * r.spec.ny * chunk;
         if (r.left == 0) {
@@ -346,8 +383,8 @@ struct Server {
       }
       if (!plain) {
         // tables hold each request's parity at build time: rebuild after an
-        // odd chunk so par stays the current buffer
-        if (chunk & 1) dirty = true;
+        // odd number of launches so par stays the current buffer
+        if (launches & 1) dirty = true;
       }
     }
   }
@@ -386,6 +423,14 @@ le2d_server *le2d_server_create(size_t hbm_budget, int max_live, int plain) {
   le2d_server *h = new le2d_server;
   h->s = s;
   return h;
+}
+
+int le2d_server_set_pairs(le2d_server *h, int pair_seg) {
+  if (!h || pair_seg < 0) return le2d::set_error(le2d::ERR_ARG, "server: pairs");
+  std::lock_guard<std::mutex> lk(h->s->mu);
+  h->s->pair_seg = pair_seg;
+  h->s->dirty = true;
+  return le2d::OK;
 }
 
 int le2d_server_set_shape(le2d_server *h, int fb, int seg, int max_chunk) {
@@ -527,6 +572,7 @@ void le2d_server_destroy(le2d_server *h) {
     if (s->stream[k]) (void)hipStreamSynchronize(s->stream[k]);
     if (s->d_inst[k]) (void)hipFree(s->d_inst[k]);
     if (s->d_tiles[k]) (void)hipFree(s->d_tiles[k]);
+    if (s->d_tiles2[k]) (void)hipFree(s->d_tiles2[k]);
     if (s->stream[k]) (void)hipStreamDestroy(s->stream[k]);
   }
   if (s->arena_dev) (void)hipFree(s->arena_dev);

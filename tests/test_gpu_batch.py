@@ -51,7 +51,7 @@ def _oracle(states, steps):
 def test_batch_fp64_bit_identical_to_oracle():
     b, states = _make(np.float64)
     with b:
-        b.step(5)  # graph (2 replays) + one single launch
+        b.step(5)  # default two-step form: one graph replay (4 steps) + one single step
         refs = _oracle(states, 5)
         for k, r in enumerate(refs):
             assert np.array_equal(b.read(k), r), SIZES[k]
@@ -59,19 +59,20 @@ def test_batch_fp64_bit_identical_to_oracle():
 
 def test_batch_plain_loop_and_shapes_agree():
     res = []
-    # (window, segment, persistent launch or one workgroup per tile)
-    for shape, plain in [((128, 32, False), False), ((64, 7, True), False),
-                         ((256, 1000, True), False), ((256, 16, False), False),
+    # (window, segment, launch form); the two-step form at odd and even counts
+    for shape, plain in [((128, 32, "tiles"), False), ((64, 7, "persistent"), False),
+                         ((256, 1000, "persistent"), False), ((256, 16, "tiles"), False),
+                         ((256, 5, "pairs"), False), ((256, 64, "pairs"), False),
                          ((128, 32, None), True)]:
         b, states = _make(np.float64)
         with b:
             b.set_shape(*shape)
-            (b.step_plain if plain else b.step)(4)
+            (b.step_plain if plain else b.step)(7)
             res.append([b.read(k) for k in range(len(SIZES))])
     for other in res[1:]:
         for a, c in zip(res[0], other):
             assert np.array_equal(a, c)
-    refs = _oracle(states, 4)
+    refs = _oracle(states, 7)
     for a, r in zip(res[0], refs):
         assert np.array_equal(a, r)
 
@@ -81,7 +82,7 @@ def test_batch_autotune_keeps_results():
     with b:
         fb, seg, ms = b.autotune(1)  # advances every instance by an even count
         assert fb in (64, 128, 256) and seg in (16, 32, 64, 128) and ms > 0
-        n_tuned = 2 * 3 * 4 * 4  # 24 launch shapes x (2 warm-up + 2 timed) steps
+        n_tuned = 2 * 3 * 4 * 4 + 4 * 8  # single-step shapes x 4 steps + two-step x 8
         for k, g in enumerate(states):
             b.load(k, g)
         b.step(3)

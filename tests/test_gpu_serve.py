@@ -100,3 +100,18 @@ def test_budget_limits_admission_and_rejects_oversize():
         for i in ids:
             ts, ta, td = s.times(i)
             assert ts <= ta <= td
+
+
+def test_two_step_launches_change_nothing():
+    # odd chunk limit and odd step count: two-step launches plus single ones
+    res = []
+    for pairs in (0, 64, 5):
+        with le2d_amd.Server(1 * GB, 16) as s:
+            s.set_pairs(pairs)
+            s.set_shape(256, 16, 7)
+            ids = [s.submit(**REQ, dtype=dt, keep=True) for dt in (np.float32, np.float64)]
+            s.drain(60)
+            res.append([s.result(i) for i in ids])
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            assert np.array_equal(a, b)
