@@ -17,7 +17,8 @@
 // alone and in any mix.
 //
 // With plain = 1 the server is the serial baseline: one slot, and each
-// request runs with the single-grid kernel (k_step_xy) and its default tiles.
+// request runs with the single-grid kernels as le_step_n uses them (k_step2_xy
+// while two steps remain, then k_step_xy) and their default tiles.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -269,8 +270,9 @@ struct Server {
                        step_parity);
   }
 
-  // The serial baseline's step: the single-grid kernel on the one live request.
-  int plain_step(Req &r) {
+  // The serial baseline's launch on the one live request: the single-grid
+  // kernels as le_step_n uses them, two steps (two = true) or one.
+  int plain_step(Req &r, bool two) {
     const le2d_request &q = r.spec;
     DevGrid g;
     g.elem = q.real_size;
@@ -286,11 +288,11 @@ struct Server {
     if (q.real_size == 8) {
       SweepCoef<double> cx, cy;
       coef_from_material<double>(q.c1, q.c2, q.rho, q.dt, q.hx, q.hy, &cx, &cy);
-      return step_xy<double>(&g, cx, cy, EDGE_CLAMP);
+      return two ? step2_xy<double>(&g, cx, cy) : step_xy<double>(&g, cx, cy, EDGE_CLAMP);
     }
     SweepCoef<float> cx, cy;
     coef_from_material<float>(q.c1, q.c2, q.rho, q.dt, q.hx, q.hy, &cx, &cy);
-    return step_xy<float>(&g, cx, cy, EDGE_CLAMP);
+    return two ? step2_xy<float>(&g, cx, cy) : step_xy<float>(&g, cx, cy, EDGE_CLAMP);
   }
 
   void run() {
@@ -324,9 +326,11 @@ struct Server {
       int launches = 0;
       if (plain) {
         Req &r = set[0].empty() ? *set[1][0] : *set[0][0];
-        for (int s = 0; s < chunk && !rc; ++s) {
-          rc = plain_step(r);  // reads buf[r.cur], writes the other
+        for (int s = 0; s < chunk && !rc;) {
+          const bool two = s + 1 < chunk;
+          rc = plain_step(r, two);  // reads buf[r.cur], writes the other
           r.cur ^= 1;
+          s += two ? 2 : 1;
         }
       } else {
         if (dirty) {
