@@ -31,6 +31,12 @@ namespace le2d {
 template <typename T>
 __device__ __forceinline__ T superbee(T r) {
   const T r2 = T(2) * r;
+  if constexpr (sizeof(T) == 4) {
+    // FP32 tier (held to an error bound against FP64, not to bits): hardware
+    // min/max, the outer two as one three-way max
+    return __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(r2, 1.0f), __builtin_fminf(r, 2.0f)),
+                           0.0f);
+  }
   const T a = (T(1) > r2) ? r2 : T(1);
   const T b = (T(2) > r) ? r : T(2);
   // (a > b) ? a : b equals the hardware max here: a and b are never NaN (a
@@ -109,10 +115,21 @@ template <typename T>
 __device__ __forceinline__ void apply(T &vn, T &vt, T &sn, T &so, T &sxy, const W4<T> &w,
                                       const W4<T> &fl, const W4<T> &fr,
                                       const SweepCoef<T> &c) {
-  const T e1 = T(0.5) * face_delta(w.a, c.k1, fr.a, fl.a);
-  const T e2 = T(0.5) * face_delta(w.b, c.k1, fl.b, fr.b);
-  const T e3 = T(0.5) * face_delta(w.c, c.k2, fr.c, fl.c);
-  const T e4 = T(0.5) * face_delta(w.d, c.k2, fl.d, fr.d);
+  T e1, e2, e3, e4;
+  if constexpr (sizeof(T) == 4) {
+    // FP32 tier: the increment (u - k*(out - in)) - u is k*(in - out), and
+    // the back-transform's factor 1/2 is folded into k (uniform, hoisted)
+    const T h1 = T(0.5) * c.k1, h2 = T(0.5) * c.k2;
+    e1 = h1 * (fl.a - fr.a);
+    e2 = h1 * (fr.b - fl.b);
+    e3 = h2 * (fl.c - fr.c);
+    e4 = h2 * (fr.d - fl.d);
+  } else {
+    e1 = T(0.5) * face_delta(w.a, c.k1, fr.a, fl.a);
+    e2 = T(0.5) * face_delta(w.b, c.k1, fl.b, fr.b);
+    e3 = T(0.5) * face_delta(w.c, c.k2, fr.c, fl.c);
+    e4 = T(0.5) * face_delta(w.d, c.k2, fl.d, fr.d);
+  }
   vn = vn + (e1 + e2);
   vt = vt + (e3 + e4);
   sn = sn + (e2 - e1) * c.rhoc1;
@@ -391,8 +408,11 @@ __device__ __forceinline__ void march_pair(const T *__restrict__ src, T *__restr
   // faces, wb the invariants of row q-1, N0/N1 the raw rows in flight (as in
   // march_tile).  Stage 2: slots A2/B2/C2 hold the x-swept time-(n+1) rows
   // r, r+1, r+2 (r = q-2), G0/G1 its faces, vb the invariants of row r-1.
+  // HOLD (FP32, which has the registers): the y-invariants of every slot are
+  // kept (WA.., VA..) instead of re-formed each step.
+  constexpr bool HOLD = sizeof(T) == 4;
   Node<T> A, B, C, N0, N1, A2, B2, C2, Xm2, Xm1;
-  W4<T> F0, F1, wb, G0, G1, vb;
+  W4<T> F0, F1, wb, G0, G1, vb, WA, WB, WC, VA, VB, VC;
   int q = j0 - 2;
   {
     const W4<T> w2 = inv_y(XS(ROW(q - 2)), cy);
@@ -401,53 +421,63 @@ __device__ __forceinline__ void march_pair(const T *__restrict__ src, T *__restr
     B = XS(ROW(q + 1));
     C = XS(ROW(q + 2));
     N0 = ROW(q + 3);
-    F0 = faces(w2, wb, inv_y(A, cy), inv_y(B, cy), cy);  // face q-1 | q
+    WA = inv_y(A, cy);
+    WB = inv_y(B, cy);
+    if constexpr (HOLD) WC = inv_y(C, cy);
+    F0 = faces(w2, wb, WA, WB, cy);  // face q-1 | q
   }
   // Stage-1 step: time-(n+1) row q (in P0), its x-sweep into X, then the next
   // raw row x-swept into P0.
-#define S1(P0, P1, P2, FI, FO, NC, NN, X)                                    \
+#define S1(P0, P1, P2, W0, W1, W2, FI, FO, NC, NN, X)                        \
   {                                                                           \
     NN = ROW(q + 4);                                                          \
-    const W4<T> w0 = inv_y(P0, cy);                                           \
-    FO = faces(wb, w0, inv_y(P1, cy), inv_y(P2, cy), cy); /* q | q+1 */       \
+    const W4<T> w0 = HOLD ? W0 : inv_y(P0, cy);                               \
+    FO = faces(wb, w0, HOLD ? W1 : inv_y(P1, cy), HOLD ? W2 : inv_y(P2, cy), cy); \
     apply(P0.vy, P0.vx, P0.syy, P0.sxx, P0.sxy, w0, FI, FO, cy);              \
     wb.a = w0.a;                                                              \
     wb.c = w0.c;                                                              \
     X = XS(P0);                                                               \
     P0 = XS(NC);                                                              \
+    if constexpr (HOLD) W0 = inv_y(P0, cy);                                   \
     ++q;                                                                      \
   }
-  S1(A, B, C, F0, F1, N0, N1, Xm2)  // q = j0-2
-  S1(B, C, A, F1, F0, N1, N0, Xm1)  // q = j0-1
-  S1(C, A, B, F0, F1, N0, N1, A2)   // q = j0
-  S1(A, B, C, F1, F0, N1, N0, B2)   // q = j0+1
-  if (j0 + 1 >= ny) B2 = A2;        // row clamp of the time-(n+1) state
+  S1(A, B, C, WA, WB, WC, F0, F1, N0, N1, Xm2)  // q = j0-2
+  S1(B, C, A, WB, WC, WA, F1, F0, N1, N0, Xm1)  // q = j0-1
+  S1(C, A, B, WC, WA, WB, F0, F1, N0, N1, A2)   // q = j0
+  S1(A, B, C, WA, WB, WC, F1, F0, N1, N0, B2)   // q = j0+1
+  if (j0 + 1 >= ny) B2 = A2;                    // row clamp of the time-(n+1) state
   if (j0 - 1 < 0) Xm1 = A2;
   if (j0 - 2 < 0) Xm2 = Xm1;
   vb = inv_y(Xm1, cy);
-  G0 = faces(inv_y(Xm2, cy), vb, inv_y(A2, cy), inv_y(B2, cy), cy);  // face j0-1 | j0
+  VA = inv_y(A2, cy);
+  VB = inv_y(B2, cy);
+  G0 = faces(inv_y(Xm2, cy), vb, VA, VB, cy);  // face j0-1 | j0
   int r = j0;
   // One stage-1 step (row q = r+2 of time n+1 into Q2, clamped past the last
   // row), then one stage-2 step writing row r of time n+2.
-#define S2(P0, P1, P2, FI, FO, NC, NN, Q0, Q1, Q2, GI, GO)                   \
+#define S2(P0, P1, P2, W0, W1, W2, FI, FO, NC, NN, Q0, Q1, Q2, V0, V1, V2, GI, GO) \
   {                                                                           \
     NN = ROW(q + 4);                                                          \
-    const W4<T> w0 = inv_y(P0, cy);                                           \
-    FO = faces(wb, w0, inv_y(P1, cy), inv_y(P2, cy), cy);                     \
+    const W4<T> w0 = HOLD ? W0 : inv_y(P0, cy);                               \
+    FO = faces(wb, w0, HOLD ? W1 : inv_y(P1, cy), HOLD ? W2 : inv_y(P2, cy), cy); \
     apply(P0.vy, P0.vx, P0.syy, P0.sxx, P0.sxy, w0, FI, FO, cy);              \
     wb.a = w0.a;                                                              \
     wb.c = w0.c;                                                              \
     {                                                                         \
       const Node<T> xn = XS(P0);                                              \
-      if (q >= ny)                                                            \
+      if (q >= ny) {                                                          \
         Q2 = Q1;                                                              \
-      else                                                                    \
+        if constexpr (HOLD) V2 = V1;                                          \
+      } else {                                                                \
         Q2 = xn;                                                              \
+        if constexpr (HOLD) V2 = inv_y(Q2, cy);                               \
+      }                                                                       \
     }                                                                         \
     P0 = XS(NC);                                                              \
+    if constexpr (HOLD) W0 = inv_y(P0, cy);                                   \
     ++q;                                                                      \
-    const W4<T> v0 = inv_y(Q0, cy);                                           \
-    GO = faces(vb, v0, inv_y(Q1, cy), inv_y(Q2, cy), cy); /* r | r+1 */       \
+    const W4<T> v0 = HOLD ? V0 : inv_y(Q0, cy);                               \
+    GO = faces(vb, v0, HOLD ? V1 : inv_y(Q1, cy), HOLD ? V2 : inv_y(Q2, cy), cy); \
     apply(Q0.vy, Q0.vx, Q0.syy, Q0.sxx, Q0.sxy, v0, GI, GO, cy);              \
     if (writer) store_row_node(dst + (long)r * pitch, plane, woff, Q0);      \
     if (++r >= j1) break;                                                     \
@@ -456,12 +486,12 @@ __device__ __forceinline__ void march_pair(const T *__restrict__ src, T *__restr
   }
   // stage 1 continues at phase 4 of its six-step slot cycle, stage 2 at 0
   for (;;) {
-    S2(B, C, A, F0, F1, N0, N1, A2, B2, C2, G0, G1)
-    S2(C, A, B, F1, F0, N1, N0, B2, C2, A2, G1, G0)
-    S2(A, B, C, F0, F1, N0, N1, C2, A2, B2, G0, G1)
-    S2(B, C, A, F1, F0, N1, N0, A2, B2, C2, G1, G0)
-    S2(C, A, B, F0, F1, N0, N1, B2, C2, A2, G0, G1)
-    S2(A, B, C, F1, F0, N1, N0, C2, A2, B2, G1, G0)
+    S2(B, C, A, WB, WC, WA, F0, F1, N0, N1, A2, B2, C2, VA, VB, VC, G0, G1)
+    S2(C, A, B, WC, WA, WB, F1, F0, N1, N0, B2, C2, A2, VB, VC, VA, G1, G0)
+    S2(A, B, C, WA, WB, WC, F0, F1, N0, N1, C2, A2, B2, VC, VA, VB, G0, G1)
+    S2(B, C, A, WB, WC, WA, F1, F0, N1, N0, A2, B2, C2, VA, VB, VC, G1, G0)
+    S2(C, A, B, WC, WA, WB, F0, F1, N0, N1, B2, C2, A2, VB, VC, VA, G0, G1)
+    S2(A, B, C, WA, WB, WC, F1, F0, N1, N0, C2, A2, B2, VC, VA, VB, G1, G0)
   }
 #undef S2
 #undef S1
