@@ -64,11 +64,11 @@ int le2d_set_tiles(le_task *task, int x_rows, int y_seg, int y_block, int xy_seg
 int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block, int *xy_seg,
                    int *xy_block);
 
-/* Two full steps per pass: with seg > 0, le_step_n advances the clamped-edge
- * plain path (no material plane) two steps per kernel pass while at least two
- * remain, `seg` rows per segment; each pass reads and writes every node once
- * for two steps.  Bit-identical to single steps.  seg = 0: one step per
- * pass.  le_step always takes one step. */
+/* Two full steps per pass: with seg > 0, le_step_n advances clamped-edge
+ * tasks (with or without a material plane) two steps per kernel pass while
+ * at least two remain, `seg` rows per segment; each pass reads and writes
+ * every node once for two steps.  Bit-identical to single steps.  seg = 0:
+ * one step per pass.  le_step always takes one step. */
 int le2d_set_step_pairs(le_task *task, int seg);
 int le2d_get_step_pairs(le_task *task, int *seg);
 /* Run `passes` two-step passes, timing each launch with HIP events; mean ms

@@ -109,6 +109,10 @@ struct MatPlane {
   MatTile *d_homog = nullptr, *d_mixed = nullptr;
   int n_homog = 0, n_mixed = 0;
   int built_fb = -1, built_seg = -1, built_edge = -1;
+  // tables of the two-step pass (strips of 248 columns)
+  MatTile *d_homog2 = nullptr, *d_mixed2 = nullptr;
+  int n_homog2 = 0, n_mixed2 = 0;
+  int built_pair_seg = -1;
 };
 
 int mat_set(DevGrid *g, MatPlane *m, int count, const double *c1c2rho, const unsigned char *ids,
@@ -116,6 +120,9 @@ int mat_set(DevGrid *g, MatPlane *m, int count, const double *c1c2rho, const uns
 void mat_release(MatPlane *m);
 template <typename T>
 int step_xy_mat(DevGrid *g, MatPlane *m, int edge);
+// Two steps in one pass (clamped edges, tiles.pair_seg rows per segment).
+template <typename T>
+int step2_xy_mat(DevGrid *g, MatPlane *m);
 
 // FP64 norms of a - b over all fields: out = {||a-b||_2, ||b||_2,
 // max|a-b|, max|b|}.  a and b are device states (planes of pitch p*, plane

@@ -133,11 +133,11 @@ int run(le_task *t, int what, int edge, int steps) {
     return le2d::set_error(LE2D_ERR_STATE,
                            "single sweeps are not available with a material plane; use le_step");
   int k = 0;
-  if (what == (DO_X | DO_Y) && !s->mat.active && edge == le2d::EDGE_CLAMP &&
-      s->g.tiles.pair_seg > 0) {
+  if (what == (DO_X | DO_Y) && edge == le2d::EDGE_CLAMP && s->g.tiles.pair_seg > 0) {
     // two steps per pass while at least two remain
     for (; k + 1 < steps; k += 2) {
-      rc = le2d::step2_xy<real>(&s->g, cx, cy);
+      rc = s->mat.active ? le2d::step2_xy_mat<real>(&s->g, &s->mat)
+                         : le2d::step2_xy<real>(&s->g, cx, cy);
       if (rc) return rc;
     }
   }

@@ -45,6 +45,37 @@ __global__ __launch_bounds__(FB) void k_step_mat_mixed(const T *__restrict__ src
                                 cxt, cyt, sn);
 }
 
+// Two-step pass: single-material tiles run march_pair with their material's
+// constants, the others the node-local two-step march.
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_step2_mat_homog(const T *__restrict__ src,
+                                                          T *__restrict__ dst, int nx, int ny,
+                                                          long pitch, long plane,
+                                                          const MatTile *__restrict__ tiles,
+                                                          const SweepCoef<T> *__restrict__ cxt,
+                                                          const SweepCoef<T> *__restrict__ cyt) {
+  __shared__ W4<T> sw[256];
+  __shared__ W4<T> sf[256];
+  const MatTile tj = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const SweepCoef<T> cx = cxt[tj.mat], cy = cyt[tj.mat];
+  march_pair<T, 256>(src, dst, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cx, cy, sw, sf);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_step2_mat_mixed(const T *__restrict__ src,
+                                                       T *__restrict__ dst, int nx, int ny,
+                                                       long pitch, long plane,
+                                                       const MatTile *__restrict__ tiles,
+                                                       const unsigned char *__restrict__ ids,
+                                                       long idp,
+                                                       const SweepCoef<T> *__restrict__ cxt,
+                                                       const SweepCoef<T> *__restrict__ cyt) {
+  __shared__ Node<T> sn[256];
+  const MatTile tj = tiles[blockIdx.x];
+  march_pair_mixed<T, 256>(src, dst, ids, idp, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cxt,
+                           cyt, sn);
+}
+
 // Energy density in FP64 (plane strain, isotropic):
 //   rho/2 |v|^2 + (sxx+syy)^2 / (8 (lambda+mu)) + ((sxx-syy)^2/2 + 2 sxy^2) / (4 mu)
 // summed over the grid; one FP64 partial per workgroup, summed on the host in
@@ -166,14 +197,14 @@ static int mcheck(hipError_t e, const char *what) {
 
 void mat_release(MatPlane *m) {
   for (void *p : {(void *)m->d_ids, m->d_cx, m->d_cy, (void *)m->d_homog, (void *)m->d_mixed,
-                  (void *)m->d_etab})
+                  (void *)m->d_homog2, (void *)m->d_mixed2, (void *)m->d_etab})
     if (p) (void)hipFree(p);
   m->d_ids = nullptr;
   m->d_cx = m->d_cy = nullptr;
-  m->d_homog = m->d_mixed = nullptr;
+  m->d_homog = m->d_mixed = m->d_homog2 = m->d_mixed2 = nullptr;
   m->d_etab = nullptr;
-  m->n_homog = m->n_mixed = 0;
-  m->built_fb = m->built_seg = m->built_edge = -1;
+  m->n_homog = m->n_mixed = m->n_homog2 = m->n_mixed2 = 0;
+  m->built_fb = m->built_seg = m->built_edge = m->built_pair_seg = -1;
 }
 
 // Material and sweep constants in double (le_init_material and the sweeps'
@@ -246,24 +277,23 @@ int mat_set(DevGrid *g, MatPlane *m, int count, const double *c1c2rho, const uns
   return OK;
 }
 
-// Split the tiles of the current launch shape into single-material and mixed.
-static int mat_tiles(DevGrid *g, MatPlane *m, int edge) {
-  const int fb = g->tiles.xy_block == 64 || g->tiles.xy_block == 128 ? g->tiles.xy_block : 256;
-  const int seg = std::max(1, g->tiles.xy_seg);
-  if (m->built_fb == fb && m->built_seg == seg && m->built_edge == edge) return OK;
-  const int nx = g->nx, ny = g->ny;
-  const int strips = (nx + fb - 5) / (fb - 4);
-  std::vector<MatTile> homog, mixed;
+// Split the tiles of a strip width and segment height into single-material
+// and mixed ones by their whole read window: the strip's `width` output
+// columns with `ch` halo columns on each side, and rows j0-rh .. j1-1+rh
+// (edge-mapped).
+static void split_tiles(const MatPlane *m, int nx, int ny, int width, int ch, int seg, int rh,
+                        int edge, std::vector<MatTile> *homog, std::vector<MatTile> *mixed) {
+  const int strips = (nx + width - 1) / width;
   std::vector<int> rows;
   for (int j0 = 0; j0 < ny; j0 += seg) {
     const int j1 = std::min(j0 + seg, ny);
     rows.clear();
-    for (int p = j0 - 2; p <= j1 + 1; ++p) {
+    for (int p = j0 - rh; p <= j1 - 1 + rh; ++p) {
       int r = p < 0 ? 0 : (p < ny ? p : (edge == EDGE_CLAMP ? ny - 1 : std::max(p - 5, 0)));
       rows.push_back(r);
     }
     for (int s = 0; s < strips; ++s) {
-      const int xa = std::max(s * (fb - 4) - 2, 0), xb = std::min(s * (fb - 4) + fb - 3, nx - 1);
+      const int xa = std::max(s * width - ch, 0), xb = std::min(s * width + width - 1 + ch, nx - 1);
       const unsigned char first = m->ids[(long)rows[0] * nx + xa];
       bool one = true;
       for (size_t q = 0; q < rows.size() && one; ++q) {
@@ -274,33 +304,52 @@ static int mat_tiles(DevGrid *g, MatPlane *m, int edge) {
             break;
           }
       }
-      (one ? homog : mixed).push_back({s, j0, j1, one ? (int)first : -1, j0 / seg});
+      (one ? homog : mixed)->push_back({s, j0, j1, one ? (int)first : -1, j0 / seg});
     }
   }
-  for (MatTile **d : {&m->d_homog, &m->d_mixed})
-    if (*d) {
-      (void)hipFree(*d);
-      *d = nullptr;
-    }
-  if (!homog.empty()) {
-    if (hipMalloc((void **)&m->d_homog, homog.size() * sizeof(MatTile)) != hipSuccess)
-      return set_error(ERR_NOMEM, "materials: tiles");
-    int rc = mcheck(hipMemcpy(m->d_homog, homog.data(), homog.size() * sizeof(MatTile),
-                              hipMemcpyHostToDevice), "materials: tiles");
-    if (rc) return rc;
-  }
-  if (!mixed.empty()) {
-    if (hipMalloc((void **)&m->d_mixed, mixed.size() * sizeof(MatTile)) != hipSuccess)
-      return set_error(ERR_NOMEM, "materials: tiles");
-    int rc = mcheck(hipMemcpy(m->d_mixed, mixed.data(), mixed.size() * sizeof(MatTile),
-                              hipMemcpyHostToDevice), "materials: tiles");
-    if (rc) return rc;
-  }
-  m->n_homog = (int)homog.size();
-  m->n_mixed = (int)mixed.size();
+}
+
+static int upload_tiles(const std::vector<MatTile> &v, MatTile **d, int *n) {
+  if (*d) (void)hipFree(*d);
+  *d = nullptr;
+  *n = 0;
+  if (v.empty()) return OK;
+  if (hipMalloc((void **)d, v.size() * sizeof(MatTile)) != hipSuccess)
+    return set_error(ERR_NOMEM, "materials: tiles");
+  int rc = mcheck(hipMemcpy(*d, v.data(), v.size() * sizeof(MatTile), hipMemcpyHostToDevice),
+                  "materials: tiles");
+  if (rc) return rc;
+  *n = (int)v.size();
+  return OK;
+}
+
+// Tables of the current single-step launch shape.
+static int mat_tiles(DevGrid *g, MatPlane *m, int edge) {
+  const int fb = g->tiles.xy_block == 64 || g->tiles.xy_block == 128 ? g->tiles.xy_block : 256;
+  const int seg = std::max(1, g->tiles.xy_seg);
+  if (m->built_fb == fb && m->built_seg == seg && m->built_edge == edge) return OK;
+  std::vector<MatTile> homog, mixed;
+  split_tiles(m, g->nx, g->ny, fb - 4, 2, seg, 2, edge, &homog, &mixed);
+  int rc = upload_tiles(homog, &m->d_homog, &m->n_homog);
+  if (!rc) rc = upload_tiles(mixed, &m->d_mixed, &m->n_mixed);
+  if (rc) return rc;
   m->built_fb = fb;
   m->built_seg = seg;
   m->built_edge = edge;
+  return OK;
+}
+
+// Tables of the two-step pass: 248-column strips read 4 halo columns and 4
+// halo rows on each side.
+static int mat_tiles2(DevGrid *g, MatPlane *m) {
+  const int seg = g->tiles.pair_seg;
+  if (m->built_pair_seg == seg) return OK;
+  std::vector<MatTile> homog, mixed;
+  split_tiles(m, g->nx, g->ny, 248, 4, seg, 4, EDGE_CLAMP, &homog, &mixed);
+  int rc = upload_tiles(homog, &m->d_homog2, &m->n_homog2);
+  if (!rc) rc = upload_tiles(mixed, &m->d_mixed2, &m->n_mixed2);
+  if (rc) return rc;
+  m->built_pair_seg = seg;
   return OK;
 }
 
@@ -344,6 +393,33 @@ int step_xy_mat(DevGrid *g, MatPlane *m, int edge) {
 
 template int step_xy_mat<double>(DevGrid *, MatPlane *, int);
 template int step_xy_mat<float>(DevGrid *, MatPlane *, int);
+
+template <typename T>
+int step2_xy_mat(DevGrid *g, MatPlane *m) {
+  if (g->elem != (int)sizeof(T)) return set_error(ERR_ARG, "step2_xy_mat: element size mismatch");
+  if (!m->active) return set_error(ERR_STATE, "step2_xy_mat: no material plane");
+  if (g->tiles.pair_seg < 1) return set_error(ERR_ARG, "step2_xy_mat: no pair segment height");
+  int rc = mat_tiles2(g, m);
+  if (rc) return rc;
+  const T *src = static_cast<const T *>(g->buf[g->cur]);
+  T *dst = static_cast<T *>(g->buf[g->cur ^ 1]);
+  const SweepCoef<T> *cx = static_cast<const SweepCoef<T> *>(m->d_cx);
+  const SweepCoef<T> *cy = static_cast<const SweepCoef<T> *>(m->d_cy);
+  if (m->n_homog2)
+    hipLaunchKernelGGL((k_step2_mat_homog<T>), dim3(m->n_homog2), dim3(256), 0, g->stream, src,
+                       dst, g->nx, g->ny, g->pitch, g->plane, m->d_homog2, cx, cy);
+  if (m->n_mixed2)
+    hipLaunchKernelGGL((k_step2_mat_mixed<T>), dim3(m->n_mixed2), dim3(256), 0, g->stream, src,
+                       dst, g->nx, g->ny, g->pitch, g->plane, m->d_mixed2, m->d_ids, m->idp, cx,
+                       cy);
+  rc = mcheck(hipGetLastError(), "step2_xy_mat launch");
+  if (rc) return rc;
+  g->cur ^= 1;
+  return OK;
+}
+
+template int step2_xy_mat<double>(DevGrid *, MatPlane *);
+template int step2_xy_mat<float>(DevGrid *, MatPlane *);
 
 int energy(DevGrid *g, const MatPlane *m, double rho, double c1, double c2, double *out) {
   double *etab = nullptr;

@@ -572,4 +572,98 @@ __device__ __forceinline__ void march_tile_mixed(const T *__restrict__ src, T *_
   }
 }
 
+// Two full steps of the node-local scheme over a tile that straddles a
+// material interface: the strip geometry, edge rules and stage order of
+// march_pair, with every node's update in its own material (as
+// march_tile_mixed).  Rows are held in shift registers: few tiles take this
+// path.  sn has FB entries.
+template <typename T, int FB>
+__device__ __forceinline__ void march_pair_mixed(const T *__restrict__ src, T *__restrict__ dst,
+                                                 const unsigned char *__restrict__ ids, long idp,
+                                                 int nx, int ny, long pitch, long plane, int strip,
+                                                 int j0, int j1,
+                                                 const SweepCoef<T> *__restrict__ cxt,
+                                                 const SweepCoef<T> *__restrict__ cyt,
+                                                 Node<T> *sn) {
+  const int t = threadIdx.x;
+  const int x0 = strip * (FB - 8);
+  const int x = x0 - 4 + t;
+  const int xc = min(max(x, 0), nx - 1);
+  const unsigned boff = (unsigned)xc * (unsigned)sizeof(T);
+  const unsigned woff = (unsigned)max(x, 0) * (unsigned)sizeof(T);
+  const bool writer = t >= 4 && t <= FB - 5 && x < nx;
+  const int tlo = max(0, 4 - x0), thi = min(FB - 1, nx + 3 - x0);
+  int nb[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) nb[k] = min(max(t - 2 + k, tlo), thi);
+  const bool inner = t >= 2 && t <= FB - 3;
+
+  // node-local x-sweep of row `row` (u: this lane's node of it)
+  auto xsweep = [&](Node<T> u, int row) -> Node<T> {
+    const int m = ids[(long)row * idp + xc];
+    sn[t] = u;
+    __syncthreads();
+    if (inner) {
+      const SweepCoef<T> c = cxt[m];
+      W4<T> w[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const Node<T> &v = sn[nb[k]];
+        w[k] = invariants(v.vx, v.sxx, v.vy, v.sxy, c);
+      }
+      const W4<T> fl = faces(w[0], w[1], w[2], w[3], c);
+      const W4<T> fr = faces(w[1], w[2], w[3], w[4], c);
+      apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w[2], fl, fr, c);
+    }
+    __syncthreads();
+    return u;
+  };
+  // node-local y-update of row j from the x-swept rows j-2 .. j+2
+  auto yupdate = [&](const Node<T> *P, int j) -> Node<T> {
+    const SweepCoef<T> c = cyt[ids[(long)j * idp + xc]];
+    W4<T> w[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) w[k] = inv_y(P[k], c);
+    const W4<T> fl = faces(w[0], w[1], w[2], w[3], c);
+    const W4<T> fr = faces(w[1], w[2], w[3], w[4], c);
+    Node<T> u = P[2];
+    apply(u.vy, u.vx, u.syy, u.sxx, u.sxy, w[2], fl, fr, c);
+    return u;
+  };
+  auto raw = [&](int p) -> Node<T> {
+    return load_row_node(src + (long)ypos<EDGE_CLAMP>(p, ny) * pitch, plane, boff);
+  };
+
+  // stage 1: P[k] = x-swept time-n row at position q-2+k (clamped rows)
+  Node<T> P[5], Q[5];
+  int q = j0 - 2;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) P[k] = xsweep(raw(q - 2 + k), ypos<EDGE_CLAMP>(q - 2 + k, ny));
+  // time-(n+1) row q x-swept for stage 2 (meaningful for 0 <= q < ny)
+  auto next = [&]() -> Node<T> {
+    const int row = min(max(q, 0), ny - 1);
+    const Node<T> y1 = yupdate(P, row);
+    const Node<T> x2 = xsweep(y1, row);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) P[k] = P[k + 1];
+    P[4] = xsweep(raw(q + 3), ypos<EDGE_CLAMP>(q + 3, ny));
+    ++q;
+    return x2;
+  };
+  // stage 2: Q[k] = x-swept time-(n+1) row at position r-2+k, clamped
+#pragma unroll
+  for (int k = 0; k < 4; ++k) Q[k] = next();  // positions j0-2 .. j0+1
+  if (j0 + 1 >= ny) Q[3] = Q[2];
+  if (j0 - 1 < 0) Q[1] = Q[2];
+  if (j0 - 2 < 0) Q[0] = Q[1];
+  for (int r = j0; r < j1; ++r) {
+    const Node<T> xn = next();  // position r+2
+    Q[4] = r + 2 >= ny ? Q[3] : xn;
+    const Node<T> u = yupdate(Q, r);
+    if (writer) store_row_node(dst + (long)r * pitch, plane, woff, u);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) Q[k] = Q[k + 1];
+  }
+}
+
 }  // namespace le2d

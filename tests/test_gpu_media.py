@@ -161,3 +161,30 @@ def test_diff_norms_across_libraries():
         assert n[2] == pytest.approx(np.abs(d).max(), rel=1e-12)
         assert n[3] == pytest.approx(np.abs(ga).max(), rel=1e-12)
         assert n[0] / n[1] < 1e-5
+
+
+@pytest.mark.parametrize("shape,kind", CASES + [((517, 67), "checker"), ((9, 7), "blobs")])
+def test_heterogeneous_pairs_equal_single_steps(shape, kind):
+    # two steps per pass (single-material and node-local mixed tiles) against
+    # one step per pass and the oracle, odd step count
+    nx, ny = shape
+    rng = np.random.default_rng(7 * nx + ny)
+    tab = [(2.0, 1.0, 1.0), (3.1, 1.6, 2.4), (1.4, 0.8, 1.2)]
+    if kind == "checker":
+        ids = (np.add.outer(np.arange(ny) // 5, np.arange(nx) // 7) % 3).astype(np.uint8)
+    else:
+        ids = _ids(nx, ny, kind, rng)
+    g0 = rng.standard_normal((ny, nx, 5))
+    dt, h = 0.45 / 3.1, (1.0, 0.9)
+    res = []
+    for seg in (0, 1, 6, 64):
+        with _task(g0, dt, h, (2.0, 1.0, 1.0)) as t:
+            t.set_materials(tab, ids)
+            t.set_step_pairs(seg)
+            t.step_n(5)
+            res.append(t.grid().copy())
+    for r in res[1:]:
+        assert np.array_equal(r, res[0])
+    ref = g0.copy()
+    OD.run_mat(ref, dt, h, tab, ids, 5)
+    assert np.array_equal(res[0], ref)
