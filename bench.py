@@ -163,6 +163,26 @@ def host_boundary_report(le2d_amd, n=8192, steps=100, trips=3):
                                     "Gnode_per_s": round(n * n * trips / el2 / 1e9, 3)}}
 
 
+def slab_report(le2d_amd, n=8192, parts=8, steps=400):
+    """One GPU's share of the grid split into `parts` row slabs (the
+    decomposition a multi-GPU run would use): the same path on an n x n/parts
+    grid, FP64 and FP32."""
+    out = {"workload": f"{n}x{n // parts} slab (1/{parts} of {n}x{n}), disk r=64, "
+                       f"{steps} steps through le_step_n"}
+    for dt, name in ((np.float64, "fp64"), (np.float32, "fp32")):
+        with le2d_amd.Task(n, n // parts, 0.25, (1.0, 1.0), (2.0, 1.0, 1.0), dtype=dt) as t:
+            t.set_ball((n / 2, n / parts / 2), 64.0, 1.0)
+            t.step_n(10)
+            t.wait()
+            t0 = time.perf_counter()
+            t.step_n(steps)
+            t.wait()
+            el = time.perf_counter() - t0
+        out[name] = {"Gnode_per_s": round(n * (n // parts) * steps / el / 1e9, 3),
+                     "ms_per_step": round(el / steps * 1e3, 4)}
+    return out
+
+
 def cpu_baseline(n, budget_s=12.0, single_budget_s=6.0):
     """Reference simple/ path (oracle/_ref, compiled from the reference's own
     sources) if present, else the oracle port; FP64, same grid and disk, a
@@ -455,6 +475,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop leg")
     ap.add_argument("--no-ref", action="store_true", help="skip the 512x512 reference-case leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe) leg")
+    ap.add_argument("--no-slab", action="store_true", help="skip the one-GPU-slab leg")
     ap.add_argument("--serve-window", type=float, default=1.0,
                     help="seconds of Poisson arrivals in the serving leg")
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -502,6 +523,7 @@ def main():
         batched = batch_report(le2d_amd, steps=args.batch_steps)
     ref_case = None if args.no_ref else reference_case_report(le2d_amd)
     host = None if args.no_host else host_boundary_report(le2d_amd, n)
+    slab = None if args.no_slab else slab_report(le2d_amd, n)
     tier = None
     if not args.no_tier:
         tier = tier_report(le2d_amd, steps=args.tier_steps)
@@ -530,6 +552,7 @@ def main():
         ("fp32" if main_dt == np.float64 else "fp64"): other,
         "reference_case": ref_case,
         "host_boundary": host,
+        "slab": slab,
         "batched": batched,
         "tier": tier,
         "serving": serving,

@@ -71,6 +71,13 @@ int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block, int *xy
  * one step per pass.  le_step always takes one step. */
 int le2d_set_step_pairs(le_task *task, int seg);
 int le2d_get_step_pairs(le_task *task, int *seg);
+/* Time the two-step pass at segment heights 16 .. 128 on the task's grid and
+ * keep the fastest.  The state does not advance: each candidate reads the
+ * current state and writes the idle buffer.  le_step_n does this once per
+ * grid shape and process on its first call of 8 or more steps, unless the
+ * caller has set a height (le2d_set_step_pairs), a material plane is set or
+ * the environment has LE2D_TUNE=0. */
+int le2d_tune_step_pairs(le_task *task, int *seg);
 /* Run `passes` two-step passes, timing each launch with HIP events; mean ms
  * of one pass (two steps).  Needs seg > 0 and an ST_AOS task. */
 int le2d_time_pairs(le_task *task, int passes, double *ms_pass);

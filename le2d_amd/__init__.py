@@ -34,6 +34,7 @@ _EXT = {
     "le2d_time_steps": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "le2d_set_step_pairs": (["T", ctypes.c_int], ctypes.c_int),
     "le2d_get_step_pairs": (["T", ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "le2d_tune_step_pairs": (["T", ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "le2d_time_pairs": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "le2d_copy_probe": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
                         ctypes.c_int),
@@ -261,6 +262,13 @@ class Task:
     def set_step_pairs(self, seg):
         """Rows per segment of the two-step pass used by step_n (0: off)."""
         self._check(self.lib.le2d_set_step_pairs(self.ref, seg), "set_step_pairs")
+
+    def tune_step_pairs(self):
+        """Time the two-step pass at several segment heights and keep the
+        fastest (the state does not advance); returns the height."""
+        v = ctypes.c_int()
+        self._check(self.lib.le2d_tune_step_pairs(self.ref, ctypes.byref(v)), "tune_step_pairs")
+        return v.value
 
     def get_step_pairs(self):
         v = ctypes.c_int()

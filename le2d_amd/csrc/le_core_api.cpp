@@ -15,7 +15,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <unordered_map>
 
 #include "engine.h"
@@ -37,10 +39,14 @@ struct TaskState {
   bool owned = true;        // mirror allocated by le_init_task
   int mode = LE2D_SYNC_WAIT;
   hipStream_t stream = nullptr;
+  bool pairs_fixed = false;  // segment height set by the caller or tuned
 };
 
 std::mutex g_mu;
 std::unordered_map<const real *, TaskState *> g_tasks;
+// tuned two-step segment heights per (element size, nx, ny), for the process
+std::mutex g_pair_mu;
+std::map<std::tuple<int, int, int>, int> g_pair_seg;
 
 int env_mode() {
   const char *s = getenv("LE2D_SYNC");
@@ -124,6 +130,58 @@ SweepCoef<real> coef(const le_task *t, real h) {
 enum { DO_X = 1, DO_Y = 2 };
 
 // Queue sweeps on the task's stream, then honour the sync mode.
+// Two-step segment height for this grid: each candidate is timed out of
+// place -- the pass reads the current state and writes the idle buffer (scratch
+// between steps), and the buffer swap is undone -- so the state does not
+// advance.  The fastest is kept, and remembered for the grid shape.
+int tune_pairs(le_task *t, TaskState *s) {
+  const auto key = std::make_tuple((int)sizeof(real), s->g.nx, s->g.ny);
+  {
+    std::lock_guard<std::mutex> lk(g_pair_mu);
+    auto it = g_pair_seg.find(key);
+    if (it != g_pair_seg.end()) {
+      s->g.tiles.pair_seg = it->second;
+      s->pairs_fixed = true;
+      return LE2D_OK;
+    }
+  }
+  const SweepCoef<real> cx = coef(t, t->h.x), cy = coef(t, t->h.y);
+  const int cand[] = {16, 24, 32, 48, 64, 96, 128};
+  hipEvent_t ev[2];
+  for (auto &e : ev) (void)hipEventCreate(&e);
+  int best = s->g.tiles.pair_seg, rc = LE2D_OK;
+  float best_ms = 1e30f;
+  for (int seg : cand) {
+    if (seg > s->g.ny && seg != cand[0]) break;  // taller segments are all one
+    s->g.tiles.pair_seg = seg;
+    for (int k = 0; k < 3 && !rc; ++k) {
+      if (k == 1) (void)hipEventRecord(ev[0], s->stream);
+      rc = le2d::step2_xy<real>(&s->g, cx, cy);
+      s->g.cur ^= 1;  // undo the swap: the state has not moved
+    }
+    (void)hipEventRecord(ev[1], s->stream);
+    if (rc || hipEventSynchronize(ev[1]) != hipSuccess) break;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = seg;
+    }
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  s->g.tiles.pair_seg = best;
+  s->pairs_fixed = true;
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_pair_mu);
+  g_pair_seg[key] = best;
+  return LE2D_OK;
+}
+
+bool tuning_enabled() {
+  const char *e = getenv("LE2D_TUNE");
+  return !(e && !strcmp(e, "0"));
+}
+
 int run(le_task *t, int what, int edge, int steps) {
   TaskState *s = find(t);
   int rc = ensure_device(t, s);
@@ -133,6 +191,11 @@ int run(le_task *t, int what, int edge, int steps) {
     return le2d::set_error(LE2D_ERR_STATE,
                            "single sweeps are not available with a material plane; use le_step");
   int k = 0;
+  if (what == (DO_X | DO_Y) && edge == le2d::EDGE_CLAMP && s->g.tiles.pair_seg > 0 &&
+      !s->pairs_fixed && !s->mat.active && steps >= 8 && tuning_enabled()) {
+    rc = tune_pairs(t, s);
+    if (rc) return rc;
+  }
   if (what == (DO_X | DO_Y) && edge == le2d::EDGE_CLAMP && s->g.tiles.pair_seg > 0) {
     // two steps per pass while at least two remain
     for (; k + 1 < steps; k += 2) {
@@ -470,6 +533,23 @@ int le2d_set_step_pairs(le_task *task, int seg) {
   if (rc) return rc;
   if (seg < 0) return le2d::set_error(LE2D_ERR_ARG, "step pairs: segment %d", seg);
   s->g.tiles.pair_seg = seg;
+  s->pairs_fixed = true;
+  return LE2D_OK;
+}
+
+int le2d_tune_step_pairs(le_task *task, int *seg) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  if (s->g.tiles.pair_seg < 1) s->g.tiles.pair_seg = 64;
+  s->pairs_fixed = false;
+  {
+    std::lock_guard<std::mutex> lk(g_pair_mu);
+    g_pair_seg.erase(std::make_tuple((int)sizeof(real), s->g.nx, s->g.ny));
+  }
+  rc = tune_pairs(task, s);
+  if (rc) return rc;
+  if (seg) *seg = s->g.tiles.pair_seg;
   return LE2D_OK;
 }
 

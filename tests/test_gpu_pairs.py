@@ -55,3 +55,19 @@ def test_pairs_disk_workload_and_timing_entry():
         assert t.time_pairs(2) > 0  # advances 4 more steps
     Oracle(np.float64).run(ref, 0.25, (1.0, 1.0), (2.0, 1.0, 1.0), 10)
     assert np.array_equal(got, ref)
+
+
+def test_tuning_leaves_the_state_alone():
+    rng = np.random.default_rng(11)
+    g0 = rng.standard_normal((300, 700, 5))
+    with le2d_amd.Task(700, 300, *ARGS) as t:
+        t.load(g0)
+        t.step_n(3)
+        before = t.grid().copy()
+        seg = t.tune_step_pairs()
+        assert seg in (16, 24, 32, 48, 64, 96, 128)
+        assert t.get_step_pairs() == seg
+        assert np.array_equal(t.grid(), before)
+        t.step_n(9)
+        got = t.grid().copy()
+    assert np.array_equal(got, _run(g0, np.float64, 0, 12))
