@@ -176,8 +176,8 @@ int le2d_batch_wait(le2d_batch *b);
  * launch form unchanged); le2d_batch_set_launch also picks the form:
  * LE2D_BATCH_TILES, one workgroup per tile; LE2D_BATCH_PERSISTENT, resident
  * workgroups walk the tile table; LE2D_BATCH_PAIRS, one workgroup per tile
- * and two steps per launch (window 256; an odd step count ends with one
- * single-step launch).  All forms give the same results. */
+ * and two steps per launch (window 128 or 256; an odd step count ends with
+ * one single-step launch).  All forms give the same results. */
 #define LE2D_BATCH_TILES 0
 #define LE2D_BATCH_PERSISTENT 1
 #define LE2D_BATCH_PAIRS 2

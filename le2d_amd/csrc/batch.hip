@@ -46,7 +46,7 @@ struct Shape {
   int fb, seg;
   int form;  // FORM_TILES: one workgroup per tile; FORM_PERSISTENT: resident
              // workgroups walk the table; FORM_PAIRS: one workgroup per tile of
-             // the two-step table, two steps per launch (fb 256)
+             // the two-step table, two steps per launch (fb 128 or 256)
 };
 
 std::mutex g_tune_mu;
@@ -168,7 +168,7 @@ struct BatchImpl {
   }
 
   int build_tiles(Shape s) {
-    if (s.form == FORM_PAIRS) s.fb = 256;
+    if (s.form == FORM_PAIRS && s.fb != 128) s.fb = 256;
     int rc = upload_table(s.form == FORM_PAIRS ? s.fb - 8 : s.fb - 4, s.seg, &d_tiles, &ntiles);
     if (rc) return rc;
     if (s.form == FORM_PAIRS) {
@@ -199,23 +199,33 @@ struct BatchImpl {
     if (shape.form == FORM_PERSISTENT)
       hipLaunchKernelGGL((k_step_batch<T, FB>), dim3(blocks), dim3(FB), 0, stream, d, d_tiles,
                          ntiles, par);
-    else if (shape.form == FORM_PAIRS)
-      hipLaunchKernelGGL((k_step_batch_pairs<T, 256>), dim3(ntiles), dim3(256), 0, stream, d,
-                         d_tiles, par);
+    else if (shape.form == FORM_PAIRS) {
+      if constexpr (FB >= 128)
+        hipLaunchKernelGGL((k_step_batch_pairs<T, FB>), dim3(ntiles), dim3(FB), 0, stream, d,
+                           d_tiles, par);
+    }
     else
       hipLaunchKernelGGL((k_step_batch_tiles<T, FB>), dim3(ntiles), dim3(FB), 0, stream, d,
                          d_tiles, par);
   }
 
-  // FORM_PAIRS: one single step (the odd remainder of a step count).
+  // FORM_PAIRS: one single step (the odd remainder of a step count), same
+  // window width.
+  template <typename T, int FB>
+  void launch_single_t(int par) {
+    hipLaunchKernelGGL((k_step_batch_tiles<T, FB>), dim3(ntiles1), dim3(FB), 0, stream,
+                       static_cast<const InstDesc<T> *>(d_inst), d_tiles1, par);
+  }
+
   void launch_single(int par) {
     if (ntiles1 <= 0) return;
-    if (elem == 8)
-      hipLaunchKernelGGL((k_step_batch_tiles<double, 256>), dim3(ntiles1), dim3(256), 0, stream,
-                         static_cast<const InstDesc<double> *>(d_inst), d_tiles1, par);
-    else
-      hipLaunchKernelGGL((k_step_batch_tiles<float, 256>), dim3(ntiles1), dim3(256), 0, stream,
-                         static_cast<const InstDesc<float> *>(d_inst), d_tiles1, par);
+    if (elem == 8) {
+      if (shape.fb == 128) launch_single_t<double, 128>(par);
+      else launch_single_t<double, 256>(par);
+    } else {
+      if (shape.fb == 128) launch_single_t<float, 128>(par);
+      else launch_single_t<float, 256>(par);
+    }
   }
 
   void launch(int par) {
@@ -431,7 +441,7 @@ int le2d_batch_wait(le2d_batch *b) {
 int le2d_batch_set_launch(le2d_batch *b, int fb, int seg, int form) {
   if (!b || !b->impl.committed || (fb != 64 && fb != 128 && fb != 256) || seg < 1 ||
       form < le2d::FORM_TILES || form > le2d::FORM_PAIRS ||
-      (form == le2d::FORM_PAIRS && fb != 256))
+      (form == le2d::FORM_PAIRS && fb == 64))
     return le2d::set_error(le2d::ERR_ARG, "batch: shape %d %d form %d", fb, seg, form);
   return b->impl.build_tiles({fb, seg, form});
 }
@@ -444,8 +454,8 @@ int le2d_batch_get_launch(le2d_batch *b, int *form) {
 
 int le2d_batch_set_shape(le2d_batch *b, int fb, int seg) {
   if (!b) return le2d::ERR_ARG;
-  const int form = b->impl.shape.form == le2d::FORM_PAIRS && fb != 256 ? le2d::FORM_TILES
-                                                                        : b->impl.shape.form;
+  const int form = b->impl.shape.form == le2d::FORM_PAIRS && fb == 64 ? le2d::FORM_TILES
+                                                                       : b->impl.shape.form;
   return le2d_batch_set_launch(b, fb, seg, form);
 }
 
@@ -494,7 +504,8 @@ int le2d_batch_autotune(le2d_batch *b, int trials, int *best_fb, int *best_seg, 
   for (int form : {le2d::FORM_TILES, le2d::FORM_PERSISTENT})
     for (int fb : fbs)
       for (int seg : segs) cand.push_back({fb, seg, form});
-  for (int seg : segs) cand.push_back({256, seg, le2d::FORM_PAIRS});
+  for (int fb : {128, 256})
+    for (int seg : segs) cand.push_back({fb, seg, le2d::FORM_PAIRS});
   le2d::Shape best{B.shape};
   double best_t = 1e30;
   for (const le2d::Shape &c : cand) {

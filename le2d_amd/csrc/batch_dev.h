@@ -62,7 +62,7 @@ __global__ __launch_bounds__(FB, 3) void k_step_batch_tiles(const InstDesc<T> *_
                                     tj.strip, tj.j0, tj.j1, tj.seg, d.cx, d.cy, sw, sf);
 }
 
-// One workgroup per tile of the two-step table (strips of 248 columns): each
+// One workgroup per tile of the two-step table (strips of FB-8 columns): each
 // launch advances every instance by two steps (march_pair).
 template <typename T, int FB>
 __global__ __launch_bounds__(FB, 2) void k_step_batch_pairs(const InstDesc<T> *__restrict__ inst,

@@ -63,6 +63,7 @@ def test_batch_plain_loop_and_shapes_agree():
     for shape, plain in [((128, 32, "tiles"), False), ((64, 7, "persistent"), False),
                          ((256, 1000, "persistent"), False), ((256, 16, "tiles"), False),
                          ((256, 5, "pairs"), False), ((256, 64, "pairs"), False),
+                         ((128, 9, "pairs"), False),
                          ((128, 32, None), True)]:
         b, states = _make(np.float64)
         with b:
@@ -82,7 +83,7 @@ def test_batch_autotune_keeps_results():
     with b:
         fb, seg, ms = b.autotune(1)  # advances every instance by an even count
         assert fb in (64, 128, 256) and seg in (16, 32, 64, 128) and ms > 0
-        n_tuned = 2 * 3 * 4 * 4 + 4 * 8  # single-step shapes x 4 steps + two-step x 8
+        n_tuned = 2 * 3 * 4 * 4 + 2 * 4 * 8  # single-step shapes x 4 steps + two-step x 8
         for k, g in enumerate(states):
             b.load(k, g)
         b.step(3)
