@@ -71,8 +71,12 @@ int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block, int *xy
  * one step per pass.  le_step always takes one step. */
 int le2d_set_step_pairs(le_task *task, int seg);
 int le2d_get_step_pairs(le_task *task, int *seg);
-/* Time the two-step pass at segment heights 16 .. 128 on the task's grid and
- * keep the fastest.  The state does not advance: each candidate reads the
+/* Lanes per two-step workgroup, 128 or 256 (it writes that many columns
+ * less eight); 256 by default. */
+int le2d_set_step_pair_window(le_task *task, int lanes);
+int le2d_get_step_pair_window(le_task *task, int *lanes);
+/* Time the two-step pass at segment heights 16 .. 128 and windows of 128 and
+ * 256 lanes on the task's grid and keep the fastest.  The state does not advance: each candidate reads the
  * current state and writes the idle buffer.  le_step_n does this once per
  * grid shape and process on its first call of 8 or more steps, unless the
  * caller has set a height (le2d_set_step_pairs), a material plane is set or

@@ -34,6 +34,8 @@ _EXT = {
     "le2d_time_steps": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "le2d_set_step_pairs": (["T", ctypes.c_int], ctypes.c_int),
     "le2d_get_step_pairs": (["T", ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "le2d_set_step_pair_window": (["T", ctypes.c_int], ctypes.c_int),
+    "le2d_get_step_pair_window": (["T", ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "le2d_tune_step_pairs": (["T", ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "le2d_time_pairs": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "le2d_copy_probe": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
@@ -263,9 +265,20 @@ class Task:
         """Rows per segment of the two-step pass used by step_n (0: off)."""
         self._check(self.lib.le2d_set_step_pairs(self.ref, seg), "set_step_pairs")
 
+    def set_step_pair_window(self, lanes):
+        """Lanes per two-step workgroup: 128 or 256 (writes lanes - 8 columns)."""
+        self._check(self.lib.le2d_set_step_pair_window(self.ref, lanes), "set_step_pair_window")
+
+    def get_step_pair_window(self):
+        v = ctypes.c_int()
+        self._check(self.lib.le2d_get_step_pair_window(self.ref, ctypes.byref(v)),
+                    "get_step_pair_window")
+        return v.value
+
     def tune_step_pairs(self):
-        """Time the two-step pass at several segment heights and keep the
-        fastest (the state does not advance); returns the height."""
+        """Time the two-step pass at several segment heights and both window
+        widths and keep the fastest (the state does not advance); returns the
+        height (get_step_pair_window gives the width)."""
         v = ctypes.c_int()
         self._check(self.lib.le2d_tune_step_pairs(self.ref, ctypes.byref(v)), "tune_step_pairs")
         return v.value

@@ -36,6 +36,7 @@ struct Tiles {
   int xy_seg;   // rows per segment of the fused x+y step
   int xy_block; // lanes per fused-step workgroup (64, 128, 256); writes xy_block-4 columns
   int pair_seg; // rows per segment of the two-step pass (0: one step per pass)
+  int pair_block; // lanes per two-step workgroup (128 or 256); writes pair_block-8 columns
 };
 
 struct DevGrid {
@@ -47,7 +48,7 @@ struct DevGrid {
   int cur = 0;         // buffer that holds the current state
   hipStream_t stream = nullptr;
   int device = 0;
-  Tiles tiles{4, 128, 256, 64, 128, 0};
+  Tiles tiles{4, 128, 256, 64, 128, 0, 256};
 };
 
 // Error reporting: every function returns 0 on success or a nonzero code and

@@ -215,15 +215,25 @@ Tiles default_tiles(int elem, int nx, int ny) {
   if (seg > ny) seg = ny;
   t.y_seg = (int)seg;
   (void)elem;
-  // two-step pass (le_step_n): 64-row segments where the grid still gives
-  // ~2048 workgroups of 248 columns, shorter (down to 16) on smaller grids
-  // (measured on 8192^2: 64 rows best, 80 vs 74 Gnode-updates/s at 32, FP64)
+  // two-step pass (le_step_n): 256-lane windows (248 columns) in 64-row
+  // segments where that still gives ~4096 workgroups (8192^2: 80 Gnode/s
+  // FP64, 74 at 32 rows); on smaller grids 128-lane windows (120 columns)
+  // in segments giving ~2048 workgroups, at least 16 rows (measured FP64,
+  // Gnode/s, 256 vs 128 lanes: 4096^2 67 vs 69, 2048^2 47 vs 52, 8192 x 1024
+  // 57 vs 62)
   {
-    const long strips = (nx + 247) / 248;
-    long seg = (long)ny * strips / 2048;
-    if (seg > 64) seg = 64;
-    if (seg < 16) seg = 16;
-    t.pair_seg = (int)seg;
+    const long strips256 = (nx + 247) / 248;
+    if ((long)ny * strips256 >= 64L * 4096) {
+      t.pair_block = 256;
+      t.pair_seg = 64;
+    } else {
+      t.pair_block = 128;
+      const long strips = (nx + 119) / 120;
+      long seg = (long)ny * strips / 2048;
+      if (seg > 64) seg = 64;
+      if (seg < 16) seg = 16;
+      t.pair_seg = (int)seg;
+    }
   }
   return t;
 }

@@ -10,6 +10,7 @@
 // struct by value.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cassert>
 #include <cmath>
 #include <cstdio>
@@ -19,6 +20,7 @@
 #include <mutex>
 #include <tuple>
 #include <unordered_map>
+#include <vector>
 
 #include "engine.h"
 #include "le2d.h"
@@ -46,7 +48,7 @@ std::mutex g_mu;
 std::unordered_map<const real *, TaskState *> g_tasks;
 // tuned two-step segment heights per (element size, nx, ny), for the process
 std::mutex g_pair_mu;
-std::map<std::tuple<int, int, int>, int> g_pair_seg;
+std::map<std::tuple<int, int, int>, std::pair<int, int>> g_pair_seg;  // (rows, lanes)
 
 int env_mode() {
   const char *s = getenv("LE2D_SYNC");
@@ -129,18 +131,21 @@ SweepCoef<real> coef(const le_task *t, real h) {
 
 enum { DO_X = 1, DO_Y = 2 };
 
-// Queue sweeps on the task's stream, then honour the sync mode.
-// Two-step segment height for this grid: each candidate is timed out of
-// place -- the pass reads the current state and writes the idle buffer (scratch
-// between steps), and the buffer swap is undone -- so the state does not
-// advance.  The fastest is kept, and remembered for the grid shape.
+// Two-step segment height and window width for this grid: each candidate is
+// timed out of place -- the pass reads the current state and writes the idle
+// buffer (scratch between steps), and the buffer swap is undone -- so the
+// state does not advance.  Every candidate is timed over 4 passes, the best
+// three again over 12 (single short timings of neighbouring candidates differ
+// by less than their noise); the fastest is kept, and remembered for the grid
+// shape.
 int tune_pairs(le_task *t, TaskState *s) {
   const auto key = std::make_tuple((int)sizeof(real), s->g.nx, s->g.ny);
   {
     std::lock_guard<std::mutex> lk(g_pair_mu);
     auto it = g_pair_seg.find(key);
     if (it != g_pair_seg.end()) {
-      s->g.tiles.pair_seg = it->second;
+      s->g.tiles.pair_seg = it->second.first;
+      s->g.tiles.pair_block = it->second.second;
       s->pairs_fixed = true;
       return LE2D_OK;
     }
@@ -149,31 +154,54 @@ int tune_pairs(le_task *t, TaskState *s) {
   const int cand[] = {16, 24, 32, 48, 64, 96, 128};
   hipEvent_t ev[2];
   for (auto &e : ev) (void)hipEventCreate(&e);
-  int best = s->g.tiles.pair_seg, rc = LE2D_OK;
-  float best_ms = 1e30f;
-  for (int seg : cand) {
-    if (seg > s->g.ny && seg != cand[0]) break;  // taller segments are all one
+  int rc = LE2D_OK;
+  // mean ms of one pass at (block, seg) after one untimed pass; < 0 on error
+  auto time_one = [&](int block, int seg, int reps) -> float {
     s->g.tiles.pair_seg = seg;
-    for (int k = 0; k < 3 && !rc; ++k) {
+    s->g.tiles.pair_block = block;
+    for (int k = 0; k <= reps && !rc; ++k) {
       if (k == 1) (void)hipEventRecord(ev[0], s->stream);
       rc = le2d::step2_xy<real>(&s->g, cx, cy);
       s->g.cur ^= 1;  // undo the swap: the state has not moved
     }
     (void)hipEventRecord(ev[1], s->stream);
-    if (rc || hipEventSynchronize(ev[1]) != hipSuccess) break;
+    if (rc || hipEventSynchronize(ev[1]) != hipSuccess) {
+      if (!rc) rc = le2d::set_error(LE2D_ERR_DEVICE, "step pairs: tuning pass failed");
+      return -1.0f;
+    }
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = seg;
+    return ms / reps;
+  };
+  struct Cand {
+    float ms;
+    int block, seg;
+  };
+  std::vector<Cand> res;
+  for (int block : {256, 128})
+    for (int seg : cand) {
+      if (seg > s->g.ny && seg != cand[0]) break;  // taller segments are all one
+      const float ms = time_one(block, seg, 4);
+      if (ms < 0) break;
+      res.push_back({ms, block, seg});
+    }
+  Cand best{1e30f, 256, 64};
+  if (!rc && !res.empty()) {
+    std::sort(res.begin(), res.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+    if (res.size() > 3) res.resize(3);
+    for (const Cand &c : res) {
+      const float ms = time_one(c.block, c.seg, 12);
+      if (ms < 0) break;
+      if (ms < best.ms) best = {ms, c.block, c.seg};
     }
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
-  s->g.tiles.pair_seg = best;
+  s->g.tiles.pair_seg = best.seg;
+  s->g.tiles.pair_block = best.block;
   s->pairs_fixed = true;
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(g_pair_mu);
-  g_pair_seg[key] = best;
+  g_pair_seg[key] = {best.seg, best.block};
   return LE2D_OK;
 }
 
@@ -182,6 +210,7 @@ bool tuning_enabled() {
   return !(e && !strcmp(e, "0"));
 }
 
+// Queue sweeps on the task's stream, then honour the sync mode.
 int run(le_task *t, int what, int edge, int steps) {
   TaskState *s = find(t);
   int rc = ensure_device(t, s);
@@ -550,6 +579,25 @@ int le2d_tune_step_pairs(le_task *task, int *seg) {
   rc = tune_pairs(task, s);
   if (rc) return rc;
   if (seg) *seg = s->g.tiles.pair_seg;
+  return LE2D_OK;
+}
+
+int le2d_set_step_pair_window(le_task *task, int lanes) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  if (lanes != 128 && lanes != 256)
+    return le2d::set_error(LE2D_ERR_ARG, "step pairs: window %d", lanes);
+  s->g.tiles.pair_block = lanes;
+  s->pairs_fixed = true;
+  return LE2D_OK;
+}
+
+int le2d_get_step_pair_window(le_task *task, int *lanes) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  *lanes = s->g.tiles.pair_block;
   return LE2D_OK;
 }
 

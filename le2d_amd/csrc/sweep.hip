@@ -163,12 +163,8 @@ int step_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy, int edge
   return OK;
 }
 
-template <typename T>
-int step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy) {
-  if (g->elem != (int)sizeof(T)) return set_error(ERR_ARG, "step2_xy: element size mismatch");
-  constexpr int FB = 256;
-  const int seg = g->tiles.pair_seg;
-  if (seg < 1) return set_error(ERR_ARG, "step2_xy: no pair segment height");
+template <typename T, int FB>
+static int launch_step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy, int seg) {
   const int nstrips = (g->nx + FB - 9) / (FB - 8);
   const long nsegs = (g->ny + seg - 1) / seg;
   const long nblocks = nsegs * nstrips;
@@ -176,6 +172,17 @@ int step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy) {
   hipLaunchKernelGGL((k_step2_xy<T, FB>), dim3((unsigned)nblocks), dim3(FB), 0, g->stream,
                      static_cast<const T *>(g->buf[g->cur]), static_cast<T *>(g->buf[g->cur ^ 1]),
                      g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx, cy);
+  return OK;
+}
+
+template <typename T>
+int step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy) {
+  if (g->elem != (int)sizeof(T)) return set_error(ERR_ARG, "step2_xy: element size mismatch");
+  const int seg = g->tiles.pair_seg;
+  if (seg < 1) return set_error(ERR_ARG, "step2_xy: no pair segment height");
+  const int rc = g->tiles.pair_block == 128 ? launch_step2_xy<T, 128>(g, cx, cy, seg)
+                                            : launch_step2_xy<T, 256>(g, cx, cy, seg);
+  if (rc) return rc;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(ERR_DEVICE, "step2_xy launch: %s", hipGetErrorString(e));
   g->cur ^= 1;

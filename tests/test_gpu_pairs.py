@@ -1,6 +1,7 @@
 """Two steps per pass (le2d_set_step_pairs): bit-identical to single steps for
 every size (one and several strips, grids narrower than a window, one to a
-few rows), segment height (1 row to more than the grid) and element type,
+few rows), segment height (1 row to more than the grid), window width (128
+and 256 lanes) and element type,
 odd step counts included (the last step takes the single-step pass), and
 to the oracle in FP64."""
 import numpy as np
@@ -15,12 +16,14 @@ SEGS = (0, 1, 3, 7, 32, 1000)
 ARGS = (0.2, (1.0, 0.9), (2.0, 1.0, 1.5))
 
 
-def _run(g0, dtype, seg, steps):
+def _run(g0, dtype, seg, steps, lanes=256):
     ny, nx = g0.shape[:2]
     with le2d_amd.Task(nx, ny, *ARGS, dtype=dtype) as t:
         t.load(g0)
         t.set_step_pairs(seg)
+        t.set_step_pair_window(lanes)
         assert t.get_step_pairs() == seg
+        assert t.get_step_pair_window() == lanes
         t.step_n(steps)
         return t.grid().copy()
 
@@ -31,8 +34,9 @@ def test_pairs_equal_single_steps(dtype, nx, ny):
     rng = np.random.default_rng(1000 * nx + ny)
     g0 = rng.standard_normal((ny, nx, 5)).astype(dtype)
     single = _run(g0, dtype, 0, 5)
-    for seg in SEGS[1:]:
-        assert np.array_equal(_run(g0, dtype, seg, 5), single), seg
+    for lanes in (256, 128):
+        for seg in SEGS[1:]:
+            assert np.array_equal(_run(g0, dtype, seg, 5, lanes), single), (lanes, seg)
 
 
 def test_pairs_match_oracle():
@@ -67,6 +71,7 @@ def test_tuning_leaves_the_state_alone():
         seg = t.tune_step_pairs()
         assert seg in (16, 24, 32, 48, 64, 96, 128)
         assert t.get_step_pairs() == seg
+        assert t.get_step_pair_window() in (128, 256)
         assert np.array_equal(t.grid(), before)
         t.step_n(9)
         got = t.grid().copy()
