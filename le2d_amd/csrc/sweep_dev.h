@@ -30,21 +30,29 @@ namespace le2d {
 // max(0, max(min(1, 2r), min(2, r))).
 template <typename T>
 __device__ __forceinline__ T superbee(T r) {
-  const T r2 = T(2) * r;
+  // max(0, max(min(1, 2r), min(2, r))) = max(c, min(r, 2)) = med3(r, c, 2)
+  // with c = clamp(2r, 0, 1): for r < 0, c = 0 and min(r, 2) = r < 0; for
+  // r >= 0, c = min(2r, 1).  The clamp is the add's output modifier.
   if constexpr (sizeof(T) == 4) {
-    // FP32 tier (held to an error bound against FP64, not to bits): hardware
-    // min/max, the outer two as one three-way max
-    return __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(r2, 1.0f), __builtin_fminf(r, 2.0f)),
-                           0.0f);
+    // FP32 tier (held to an error bound against FP64, not to bits): two
+    // instructions.  A NaN r (0/0: both slopes exactly -eps) gives 0 here,
+    // the first-order flux, where the reference's formula gives 2.
+    const float c = __builtin_fminf(__builtin_fmaxf(r + r, 0.0f), 1.0f);
+    return __builtin_amdgcn_fmed3f(r, c, 2.0f);
+  } else {
+    // FP64: max(c, min(r, 2)) is the reference's value bit for bit for every
+    // r (NaN and infinities included: the clamp sends NaN to 0, min(NaN, 2)
+    // is 2) except r = -0, where the clamp gives +0 and the reference's
+    // ternaries keep -0.  For r = +-0 the result is +0 (low word 0), so
+    // taking the high word from r there restores the sign: one select of a
+    // 32-bit half instead of a 64-bit select.
+    const double c = __builtin_fmin(__builtin_fmax(r + r, 0.0), 1.0);
+    const double m = __builtin_fmax(c, __builtin_fmin(r, 2.0));
+    const unsigned long long mb = __builtin_bit_cast(unsigned long long, m);
+    const unsigned rh = (unsigned)(__builtin_bit_cast(unsigned long long, r) >> 32);
+    const unsigned hi = r == 0.0 ? rh : (unsigned)(mb >> 32);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | (mb & 0xffffffffull));
   }
-  const T a = (T(1) > r2) ? r2 : T(1);
-  const T b = (T(2) > r) ? r : T(2);
-  // (a > b) ? a : b equals the hardware max here: a and b are never NaN (a
-  // NaN r gives a = 1, b = 2), and they are zeros only together and with the
-  // same sign (a = 2r, b = r), so no signed-zero tie can pick differently.
-  const T m = __builtin_fmax(a, b);
-  // kept as a select: for r = -0 the reference returns -0
-  return (T(0) > m) ? T(0) : m;
 }
 
 // Limited flux through the face between upwind node u0 and downwind node u1;
