@@ -73,8 +73,12 @@ __device__ __forceinline__ float slope_ratio(float n, float d) {
 template <typename T>
 __device__ __forceinline__ T face_flux(T um, T u0, T u1, T hk) {
   const T eps = T(1e-6);
-  const T r = slope_ratio((u0 - um) + eps, (u1 - u0) + eps);
-  return u0 + (superbee(r) * hk) * (u1 - u0);
+  const T du = u1 - u0;
+  const T r = slope_ratio((u0 - um) + eps, du + eps);
+  if constexpr (sizeof(T) == 4)
+    return __builtin_fmaf(superbee(r) * hk, du, u0);  // FP32 tier: fused
+  else
+    return u0 + (superbee(r) * hk) * du;
 }
 
 // Increment of one invariant from its outflow and inflow face fluxes
@@ -93,13 +97,21 @@ struct W4 {
 // stress is sn (simple/le_core.c:192-218).
 template <typename T>
 __device__ __forceinline__ W4<T> invariants(T vn, T sn, T vt, T sxy, const SweepCoef<T> &c) {
-  const T n = sn * c.irhoc1;
-  const T m = sxy * c.irhoc2;
   W4<T> w;
-  w.a = vn - n;
-  w.b = vn + n;
-  w.c = vt - m;
-  w.d = vt + m;
+  if constexpr (sizeof(T) == 4) {
+    // FP32 tier: one fused multiply-add per invariant
+    w.a = __builtin_fmaf(-sn, c.irhoc1, vn);
+    w.b = __builtin_fmaf(sn, c.irhoc1, vn);
+    w.c = __builtin_fmaf(-sxy, c.irhoc2, vt);
+    w.d = __builtin_fmaf(sxy, c.irhoc2, vt);
+  } else {
+    const T n = sn * c.irhoc1;
+    const T m = sxy * c.irhoc2;
+    w.a = vn - n;
+    w.b = vn + n;
+    w.c = vt - m;
+    w.d = vt + m;
+  }
   return w;
 }
 
@@ -123,26 +135,29 @@ template <typename T>
 __device__ __forceinline__ void apply(T &vn, T &vt, T &sn, T &so, T &sxy, const W4<T> &w,
                                       const W4<T> &fl, const W4<T> &fr,
                                       const SweepCoef<T> &c) {
-  T e1, e2, e3, e4;
   if constexpr (sizeof(T) == 4) {
-    // FP32 tier: the increment (u - k*(out - in)) - u is k*(in - out), and
-    // the back-transform's factor 1/2 is folded into k (uniform, hoisted)
+    // FP32 tier: the increment (u - k*(out - in)) - u is k*(in - out); the
+    // back-transform's factor 1/2 and the stress factors are folded into
+    // uniform constants (hoisted), each field takes one fused multiply-add
     const T h1 = T(0.5) * c.k1, h2 = T(0.5) * c.k2;
-    e1 = h1 * (fl.a - fr.a);
-    e2 = h1 * (fr.b - fl.b);
-    e3 = h2 * (fl.c - fr.c);
-    e4 = h2 * (fr.d - fl.d);
+    const T da = fl.a - fr.a, db = fr.b - fl.b, dc = fl.c - fr.c, dd = fr.d - fl.d;
+    const T t1 = db - da, t2 = dd - dc;
+    vn = __builtin_fmaf(h1, da + db, vn);
+    vt = __builtin_fmaf(h2, dc + dd, vt);
+    sn = __builtin_fmaf(h1 * c.rhoc1, t1, sn);
+    so = __builtin_fmaf(h1 * c.rhoc3, t1, so);
+    sxy = __builtin_fmaf(h2 * c.rhoc2, t2, sxy);
   } else {
-    e1 = T(0.5) * face_delta(w.a, c.k1, fr.a, fl.a);
-    e2 = T(0.5) * face_delta(w.b, c.k1, fl.b, fr.b);
-    e3 = T(0.5) * face_delta(w.c, c.k2, fr.c, fl.c);
-    e4 = T(0.5) * face_delta(w.d, c.k2, fl.d, fr.d);
+    const T e1 = T(0.5) * face_delta(w.a, c.k1, fr.a, fl.a);
+    const T e2 = T(0.5) * face_delta(w.b, c.k1, fl.b, fr.b);
+    const T e3 = T(0.5) * face_delta(w.c, c.k2, fr.c, fl.c);
+    const T e4 = T(0.5) * face_delta(w.d, c.k2, fl.d, fr.d);
+    vn = vn + (e1 + e2);
+    vt = vt + (e3 + e4);
+    sn = sn + (e2 - e1) * c.rhoc1;
+    so = so + (e2 - e1) * c.rhoc3;
+    sxy = sxy + c.rhoc2 * (e4 - e3);
   }
-  vn = vn + (e1 + e2);
-  vt = vt + (e3 + e4);
-  sn = sn + (e2 - e1) * c.rhoc1;
-  so = so + (e2 - e1) * c.rhoc3;
-  sxy = sxy + c.rhoc2 * (e4 - e3);
 }
 
 // ---------------------------------------------------------------------------
