@@ -308,8 +308,8 @@ void le2d_batch_destroy(le2d_batch *b) { delete b; }
 int le2d_batch_add(le2d_batch *b, int nx, int ny, double dt, double hx, double hy, double c1,
                    double c2, double rho) {
   if (!b || b->impl.committed) return -le2d::set_error(le2d::ERR_STATE, "batch: add after commit");
-  if (nx < 3 || ny < 3 || !(dt > 0) || !(hx > 0) || !(hy > 0) || !(c1 > 0) || !(c2 > 0) ||
-      !(rho > 0))
+  if (nx < 3 || ny < 3 || !le2d::row_fits(b->impl.elem, nx) || !(dt > 0) || !(hx > 0) ||
+      !(hy > 0) || !(c1 > 0) || !(c2 > 0) || !(rho > 0))
     return -le2d::set_error(le2d::ERR_ARG, "batch: bad instance %dx%d", nx, ny);
   le2d::Inst in{};
   in.nx = nx;

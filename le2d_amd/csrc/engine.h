@@ -90,6 +90,9 @@ int step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy);
 int copy_probe(size_t bytes, int iters, double *ms);
 
 Tiles default_tiles(int elem, int nx, int ny);
+// One grid row (five padded field rows) fits the 31-bit range of a buffer
+// resource, which the row loads and stores address through.
+bool row_fits(int elem, int nx);
 
 // Heterogeneous media.  A material plane assigns each node one of `count`
 // materials (c1, c2, rho); see media.hip for the scheme.

@@ -52,6 +52,10 @@ void grid_layout(int elem, int nx, int ny, long *pitch, long *plane) {
   *plane = fp;
 }
 
+bool row_fits(int elem, int nx) {
+  return nx >= 1 && 5 * field_pitch(nx, elem) * (long)elem <= 0x7fffffffL;
+}
+
 size_t grid_bytes(int elem, int nx, int ny) {
   return (size_t)5 * field_pitch(nx, elem) * ny * elem;
 }
@@ -60,6 +64,7 @@ int grid_alloc(DevGrid *g, int elem, int nx, int ny, hipStream_t stream) {
   if ((elem != 4 && elem != 8) || nx < 3 || ny < 3)
     return set_error(ERR_ARG, "grid_alloc: need elem 4|8 and nx, ny >= 3 (got %d, %d, %d)", elem,
                      nx, ny);
+  if (!row_fits(elem, nx)) return set_error(ERR_ARG, "grid_alloc: row of %d nodes too long", nx);
   g->elem = elem;
   g->nx = nx;
   g->ny = ny;

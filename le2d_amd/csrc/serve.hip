@@ -450,7 +450,8 @@ int le2d_server_set_shape(le2d_server *h, int fb, int seg, int max_chunk) {
 
 long long le2d_server_submit(le2d_server *h, const le2d_request *q) {
   if (!h || !q) return -le2d::ERR_ARG;
-  if ((q->real_size != 4 && q->real_size != 8) || q->nx < 3 || q->ny < 3 || q->steps < 1 ||
+  if ((q->real_size != 4 && q->real_size != 8) || q->nx < 3 || q->ny < 3 ||
+      !le2d::row_fits(q->real_size, q->nx) || q->steps < 1 ||
       !(q->dt > 0) || !(q->hx > 0) || !(q->hy > 0) || !(q->c1 > 0) || !(q->c2 > 0) ||
       !(q->rho > 0))
     return -le2d::set_error(le2d::ERR_ARG, "server: bad request");

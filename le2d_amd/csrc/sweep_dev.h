@@ -99,10 +99,12 @@ template <typename T>
 __device__ __forceinline__ W4<T> invariants(T vn, T sn, T vt, T sxy, const SweepCoef<T> &c) {
   W4<T> w;
   if constexpr (sizeof(T) == 4) {
-    // FP32 tier: one fused multiply-add per invariant
-    w.a = __builtin_fmaf(-sn, c.irhoc1, vn);
+    // FP32 tier: one fused multiply-add per invariant (the negations on the
+    // uniform factors, so no lane value needs negating)
+    const T ni1 = -c.irhoc1, ni2 = -c.irhoc2;
+    w.a = __builtin_fmaf(sn, ni1, vn);
     w.b = __builtin_fmaf(sn, c.irhoc1, vn);
-    w.c = __builtin_fmaf(-sxy, c.irhoc2, vt);
+    w.c = __builtin_fmaf(sxy, ni2, vt);
     w.d = __builtin_fmaf(sxy, c.irhoc2, vt);
   } else {
     const T n = sn * c.irhoc1;
@@ -188,40 +190,62 @@ __device__ __forceinline__ Node<T> load_node(const T *__restrict__ p, long plane
   return n;
 }
 
-// Loads and stores through a wavefront-uniform row base (kept in SGPRs) and a
-// 32-bit per-lane byte offset, which maps to the SGPR-base + VGPR-offset
-// addressing of global memory instructions and keeps VGPRs free of 64-bit
-// addresses.  The offset is a column offset within one row, so it always fits.
+// Row loads and stores go through a buffer resource whose base is the row
+// (wavefront-uniform, built in SGPRs): the lane's byte offset within the field
+// row is the VGPR offset and the field's plane offset the SGPR offset, so a
+// load or store costs no per-lane address arithmetic (a global access with a
+// 64-bit per-lane address costs one 64-bit add each).  The resource covers
+// the row's five padded field rows (row_fits keeps that under 2^31 bytes);
+// every offset used is inside it.  Word 3 is the gfx9 raw-buffer format
+// (32-bit data, no swizzle).
 template <typename T>
-__device__ __forceinline__ T ld_u(const T *__restrict__ base, unsigned boff) {
-  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const T *row, long plane) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(row), (short)0,
+                                           (int)(5 * plane * (long)sizeof(T)), 0x00020000);
 }
 
 template <typename T>
-__device__ __forceinline__ void st_u(T *__restrict__ base, unsigned boff, T v) {
-  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff, T v) {
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
 
 template <typename T>
 __device__ __forceinline__ Node<T> load_row_node(const T *__restrict__ row, long plane,
                                                  unsigned boff) {
+  const __amdgpu_buffer_rsrc_t r = row_rsrc(row, plane);
+  const int ps = (int)(plane * (long)sizeof(T));
   Node<T> n;
-  n.vx = ld_u(row, boff);
-  n.vy = ld_u(row + plane, boff);
-  n.sxx = ld_u(row + 2 * plane, boff);
-  n.sxy = ld_u(row + 3 * plane, boff);
-  n.syy = ld_u(row + 4 * plane, boff);
+  n.vx = buf_ld<T>(r, boff, 0);
+  n.vy = buf_ld<T>(r, boff, ps);
+  n.sxx = buf_ld<T>(r, boff, 2 * ps);
+  n.sxy = buf_ld<T>(r, boff, 3 * ps);
+  n.syy = buf_ld<T>(r, boff, 4 * ps);
   return n;
 }
 
 template <typename T>
 __device__ __forceinline__ void store_row_node(T *__restrict__ row, long plane, unsigned boff,
                                                const Node<T> &u) {
-  st_u(row, boff, u.vx);
-  st_u(row + plane, boff, u.vy);
-  st_u(row + 2 * plane, boff, u.sxx);
-  st_u(row + 3 * plane, boff, u.sxy);
-  st_u(row + 4 * plane, boff, u.syy);
+  const __amdgpu_buffer_rsrc_t r = row_rsrc(row, plane);
+  const int ps = (int)(plane * (long)sizeof(T));
+  buf_st<T>(r, boff, 0, u.vx);
+  buf_st<T>(r, boff, ps, u.vy);
+  buf_st<T>(r, boff, 2 * ps, u.sxx);
+  buf_st<T>(r, boff, 3 * ps, u.sxy);
+  buf_st<T>(r, boff, 4 * ps, u.syy);
 }
 
 template <typename T>

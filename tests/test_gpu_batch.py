@@ -120,3 +120,15 @@ def test_device_ball_matches_host_set_ball(dtype):
             with le2d_amd.Task(nx, ny, 0.1, h, (2.0, 1.0, 1.0), dtype=dtype) as t:
                 t.set_ball(c, r, s)
                 assert np.array_equal(b.read(k), t.view()), (k, np.count_nonzero(b.read(k) != t.view()))
+
+
+def test_instance_row_too_long_is_rejected():
+    # a row of five FP64 fields over 2^31 bytes cannot be addressed by the
+    # row buffer resources: refused at add, before any allocation
+    b = le2d_amd.Batch(np.float64)
+    try:
+        with pytest.raises(le2d_amd.LE2DError):
+            b.add(60_000_000, 3, 0.25)
+        assert b.add(1000, 3, 0.25) == 0
+    finally:
+        b.close()
