@@ -71,12 +71,13 @@ int le2d_get_tiles(le_task *task, int *x_rows, int *y_seg, int *y_block, int *xy
  * one step per pass.  le_step always takes one step. */
 int le2d_set_step_pairs(le_task *task, int seg);
 int le2d_get_step_pairs(le_task *task, int *seg);
-/* Lanes per two-step workgroup, 128 or 256 (it writes that many columns
- * less eight); 256 by default. */
+/* Lanes per two-step workgroup, 64, 128 or 256 (it writes that many columns
+ * less eight).  The default is 64 lanes (one wavefront), or 256 for FP32 grids
+ * of about 8192^2 and more. */
 int le2d_set_step_pair_window(le_task *task, int lanes);
 int le2d_get_step_pair_window(le_task *task, int *lanes);
-/* Time the two-step pass at segment heights 16 .. 128 and windows of 128 and
- * 256 lanes on the task's grid and keep the fastest.  The state does not advance: each candidate reads the
+/* Time the two-step pass at segment heights 16 .. 128 and windows of 64, 128
+ * and 256 lanes on the task's grid and keep the fastest.  The state does not advance: each candidate reads the
  * current state and writes the idle buffer.  le_step_n does this once per
  * grid shape and process on its first call of 8 or more steps, unless the
  * caller has set a height (le2d_set_step_pairs), a material plane is set or
@@ -180,8 +181,8 @@ int le2d_batch_wait(le2d_batch *b);
  * launch form unchanged); le2d_batch_set_launch also picks the form:
  * LE2D_BATCH_TILES, one workgroup per tile; LE2D_BATCH_PERSISTENT, resident
  * workgroups walk the tile table; LE2D_BATCH_PAIRS, one workgroup per tile
- * and two steps per launch (window 128 or 256; an odd step count ends with
- * one single-step launch).  All forms give the same results. */
+ * and two steps per launch (an odd step count ends with one single-step
+ * launch).  All forms give the same results. */
 #define LE2D_BATCH_TILES 0
 #define LE2D_BATCH_PERSISTENT 1
 #define LE2D_BATCH_PAIRS 2
@@ -235,6 +236,9 @@ int le2d_server_set_shape(le2d_server *srv, int fb, int seg, int max_chunk);
 /* Rows per segment of the two-step launches (default 64); 0: one step per
  * launch.  Results do not change. */
 int le2d_server_set_pairs(le2d_server *srv, int pair_seg);
+/* Lanes per two-step workgroup, 64, 128 or 256; 0 (the default): 256 for
+ * FP32 requests and 64 for FP64.  Results do not change. */
+int le2d_server_set_pair_window(le2d_server *srv, int lanes);
 /* Queue a request; returns its id (>= 0) or a negative error code
  * (-LE2D_ERR_BUDGET when it can never fit). */
 long long le2d_server_submit(le2d_server *srv, const le2d_request *req);

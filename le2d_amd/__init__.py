@@ -97,6 +97,7 @@ _BATCH = {
     "le2d_server_create": ([ctypes.c_size_t, ctypes.c_int, ctypes.c_int], _VP),
     "le2d_server_set_shape": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "le2d_server_set_pairs": ([_VP, ctypes.c_int], ctypes.c_int),
+    "le2d_server_set_pair_window": ([_VP, ctypes.c_int], ctypes.c_int),
     "le2d_server_submit": ([_VP, ctypes.POINTER(le2d_request)], ctypes.c_longlong),
     "le2d_server_wait": ([_VP, ctypes.c_longlong, ctypes.c_double], ctypes.c_int),
     "le2d_server_drain": ([_VP, ctypes.c_double], ctypes.c_int),
@@ -266,7 +267,7 @@ class Task:
         self._check(self.lib.le2d_set_step_pairs(self.ref, seg), "set_step_pairs")
 
     def set_step_pair_window(self, lanes):
-        """Lanes per two-step workgroup: 128 or 256 (writes lanes - 8 columns)."""
+        """Lanes per two-step workgroup: 64, 128 or 256 (writes lanes - 8 columns)."""
         self._check(self.lib.le2d_set_step_pair_window(self.ref, lanes), "set_step_pair_window")
 
     def get_step_pair_window(self):
@@ -476,6 +477,11 @@ class Server:
     def set_pairs(self, pair_seg=64):
         """Rows per segment of the two-step launches; 0: one step per launch."""
         self._check(self.lib.le2d_server_set_pairs(self.h, pair_seg), "set_pairs")
+
+    def set_pair_window(self, lanes=0):
+        """Lanes per two-step workgroup: 64, 128 or 256 (0: 256 for FP32
+        requests, 64 for FP64)."""
+        self._check(self.lib.le2d_server_set_pair_window(self.h, lanes), "set_pair_window")
 
     def submit(self, nx, ny, steps, dtype=np.float32, dt=0.25, h=(1.0, 1.0),
                mat=(2.0, 1.0, 1.0), disk=(None, None, 10.0, 1.0), keep=False):

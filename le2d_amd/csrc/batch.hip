@@ -168,7 +168,7 @@ struct BatchImpl {
   }
 
   int build_tiles(Shape s) {
-    if (s.form == FORM_PAIRS && s.fb != 128) s.fb = 256;
+    if (s.form == FORM_PAIRS && s.fb != 64 && s.fb != 128) s.fb = 256;
     int rc = upload_table(s.form == FORM_PAIRS ? s.fb - 8 : s.fb - 4, s.seg, &d_tiles, &ntiles);
     if (rc) return rc;
     if (s.form == FORM_PAIRS) {
@@ -199,11 +199,9 @@ struct BatchImpl {
     if (shape.form == FORM_PERSISTENT)
       hipLaunchKernelGGL((k_step_batch<T, FB>), dim3(blocks), dim3(FB), 0, stream, d, d_tiles,
                          ntiles, par);
-    else if (shape.form == FORM_PAIRS) {
-      if constexpr (FB >= 128)
-        hipLaunchKernelGGL((k_step_batch_pairs<T, FB>), dim3(ntiles), dim3(FB), 0, stream, d,
-                           d_tiles, par);
-    }
+    else if (shape.form == FORM_PAIRS)
+      hipLaunchKernelGGL((k_step_batch_pairs<T, FB>), dim3(ntiles), dim3(FB), 0, stream, d,
+                         d_tiles, par);
     else
       hipLaunchKernelGGL((k_step_batch_tiles<T, FB>), dim3(ntiles), dim3(FB), 0, stream, d,
                          d_tiles, par);
@@ -220,10 +218,12 @@ struct BatchImpl {
   void launch_single(int par) {
     if (ntiles1 <= 0) return;
     if (elem == 8) {
-      if (shape.fb == 128) launch_single_t<double, 128>(par);
+      if (shape.fb == 64) launch_single_t<double, 64>(par);
+      else if (shape.fb == 128) launch_single_t<double, 128>(par);
       else launch_single_t<double, 256>(par);
     } else {
-      if (shape.fb == 128) launch_single_t<float, 128>(par);
+      if (shape.fb == 64) launch_single_t<float, 64>(par);
+      else if (shape.fb == 128) launch_single_t<float, 128>(par);
       else launch_single_t<float, 256>(par);
     }
   }
@@ -440,8 +440,7 @@ int le2d_batch_wait(le2d_batch *b) {
 
 int le2d_batch_set_launch(le2d_batch *b, int fb, int seg, int form) {
   if (!b || !b->impl.committed || (fb != 64 && fb != 128 && fb != 256) || seg < 1 ||
-      form < le2d::FORM_TILES || form > le2d::FORM_PAIRS ||
-      (form == le2d::FORM_PAIRS && fb == 64))
+      form < le2d::FORM_TILES || form > le2d::FORM_PAIRS)
     return le2d::set_error(le2d::ERR_ARG, "batch: shape %d %d form %d", fb, seg, form);
   return b->impl.build_tiles({fb, seg, form});
 }
@@ -454,9 +453,7 @@ int le2d_batch_get_launch(le2d_batch *b, int *form) {
 
 int le2d_batch_set_shape(le2d_batch *b, int fb, int seg) {
   if (!b) return le2d::ERR_ARG;
-  const int form = b->impl.shape.form == le2d::FORM_PAIRS && fb == 64 ? le2d::FORM_TILES
-                                                                       : b->impl.shape.form;
-  return le2d_batch_set_launch(b, fb, seg, form);
+  return le2d_batch_set_launch(b, fb, seg, b->impl.shape.form);
 }
 
 int le2d_batch_get_shape(le2d_batch *b, int *fb, int *seg, int *ntiles, int *blocks) {
@@ -504,7 +501,7 @@ int le2d_batch_autotune(le2d_batch *b, int trials, int *best_fb, int *best_seg, 
   for (int form : {le2d::FORM_TILES, le2d::FORM_PERSISTENT})
     for (int fb : fbs)
       for (int seg : segs) cand.push_back({fb, seg, form});
-  for (int fb : {128, 256})
+  for (int fb : fbs)
     for (int seg : segs) cand.push_back({fb, seg, le2d::FORM_PAIRS});
   le2d::Shape best{B.shape};
   double best_t = 1e30;

@@ -219,25 +219,25 @@ Tiles default_tiles(int elem, int nx, int ny) {
   if (seg < 32) seg = 32;
   if (seg > ny) seg = ny;
   t.y_seg = (int)seg;
-  (void)elem;
-  // two-step pass (le_step_n): 256-lane windows (248 columns) in 64-row
-  // segments where that still gives ~4096 workgroups (8192^2: 80 Gnode/s
-  // FP64, 74 at 32 rows); on smaller grids 128-lane windows (120 columns)
-  // in segments giving ~2048 workgroups, at least 16 rows (measured FP64,
-  // Gnode/s, 256 vs 128 lanes: 4096^2 67 vs 69, 2048^2 47 vs 52, 8192 x 1024
-  // 57 vs 62)
+  // two-step pass (le_step_n), measured on MI355X (Gnode-updates/s):
+  // one-wave windows (64 lanes, 56 columns: no barriers between waves) in
+  // the tallest of 64/32/16-row segments that still gives ~4096 workgroups
+  // (FP64 8192^2 87.8 at 64 rows against 85.1 at 128 lanes and 84.5 at 256;
+  // 8192 x 1024 71.8 at 32 rows against 65.7; 2048^2 59.8 at 16 against
+  // 54.8).  FP32, issue-bound rather than latency-bound, keeps 256-lane
+  // windows (248 columns, 3% halo) where 64-row segments of them still give
+  // ~4096 workgroups (8192^2: 229.5 against 206.8 with 64 lanes).
   {
     const long strips256 = (nx + 247) / 248;
-    if ((long)ny * strips256 >= 64L * 4096) {
+    if (elem == 4 && (long)ny * strips256 >= 64L * 4096) {
       t.pair_block = 256;
       t.pair_seg = 64;
     } else {
-      t.pair_block = 128;
-      const long strips = (nx + 119) / 120;
-      long seg = (long)ny * strips / 2048;
-      if (seg > 64) seg = 64;
-      if (seg < 16) seg = 16;
-      t.pair_seg = (int)seg;
+      t.pair_block = 64;
+      const long strips = (nx + 55) / 56;
+      int seg = 64;
+      while (seg > 16 && (long)((ny + seg - 1) / seg) * strips < 4096) seg /= 2;
+      t.pair_seg = seg;
     }
   }
   return t;

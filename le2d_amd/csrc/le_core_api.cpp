@@ -178,7 +178,7 @@ int tune_pairs(le_task *t, TaskState *s) {
     int block, seg;
   };
   std::vector<Cand> res;
-  for (int block : {256, 128})
+  for (int block : {256, 128, 64})
     for (int seg : cand) {
       if (seg > s->g.ny && seg != cand[0]) break;  // taller segments are all one
       const float ms = time_one(block, seg, 4);
@@ -586,7 +586,7 @@ int le2d_set_step_pair_window(le_task *task, int lanes) {
   TaskState *s = find(task);
   int rc = ensure_device(task, s);
   if (rc) return rc;
-  if (lanes != 128 && lanes != 256)
+  if (lanes != 64 && lanes != 128 && lanes != 256)
     return le2d::set_error(LE2D_ERR_ARG, "step pairs: window %d", lanes);
   s->g.tiles.pair_block = lanes;
   s->pairs_fixed = true;

@@ -106,6 +106,11 @@ struct Server {
   int max_chunk = 16;
   int fb = 256, seg = 16;
   int pair_seg = 64;  // two-step launches (segment rows); 0: one step per launch
+  // lanes per two-step workgroup (64, 128 or 256); 0: 256 for FP32 requests,
+  // which are issue-bound, 64 (one wavefront, no barriers between waves)
+  // for FP64, which are latency-bound
+  int pair_fb = 0;
+  int pair_lanes(int si) const { return pair_fb ? pair_fb : (si == 1 ? 64 : 256); }
   char *arena_dev = nullptr;
   Arena arena;
   std::mutex mu;
@@ -217,8 +222,9 @@ struct Server {
       for (int j0 = 0; j0 < q.ny; j0 += sg)
         for (int s = 0; s < strips; ++s)
           tiles.push_back({(int)i, s, j0, std::min(j0 + sg, q.ny), j0 / sg});
-      if (pair_seg > 0) {  // strips of 248 output columns for k_step_batch_pairs<T, 256>
-        const int strips2 = (q.nx + 247) / 248;
+      if (pair_seg > 0) {  // strips of lanes - 8 output columns (k_step_batch_pairs)
+        const int lanes = pair_lanes(si);
+        const int strips2 = (q.nx + lanes - 9) / (lanes - 8);
         const int sg2 = std::max(1, std::min(pair_seg, q.ny));
         for (int j0 = 0; j0 < q.ny; j0 += sg2)
           for (int s = 0; s < strips2; ++s)
@@ -265,9 +271,18 @@ struct Server {
   template <typename T>
   void launch_pairs(int si, int step_parity) {
     if (ntiles2[si] <= 0) return;
-    hipLaunchKernelGGL((k_step_batch_pairs<T, 256>), dim3((unsigned)ntiles2[si]), dim3(256), 0,
-                       stream[si], static_cast<const InstDesc<T> *>(d_inst[si]), d_tiles2[si],
-                       step_parity);
+    const dim3 grid((unsigned)ntiles2[si]);
+    const InstDesc<T> *d = static_cast<const InstDesc<T> *>(d_inst[si]);
+    const int lanes = pair_lanes(si);
+    if (lanes == 64)
+      hipLaunchKernelGGL((k_step_batch_pairs<T, 64>), grid, dim3(64), 0, stream[si], d,
+                         d_tiles2[si], step_parity);
+    else if (lanes == 128)
+      hipLaunchKernelGGL((k_step_batch_pairs<T, 128>), grid, dim3(128), 0, stream[si], d,
+                         d_tiles2[si], step_parity);
+    else
+      hipLaunchKernelGGL((k_step_batch_pairs<T, 256>), grid, dim3(256), 0, stream[si], d,
+                         d_tiles2[si], step_parity);
   }
 
   // The serial baseline's launch on the one live request: the single-grid
@@ -433,6 +448,15 @@ int le2d_server_set_pairs(le2d_server *h, int pair_seg) {
   if (!h || pair_seg < 0) return le2d::set_error(le2d::ERR_ARG, "server: pairs");
   std::lock_guard<std::mutex> lk(h->s->mu);
   h->s->pair_seg = pair_seg;
+  h->s->dirty = true;
+  return le2d::OK;
+}
+
+int le2d_server_set_pair_window(le2d_server *h, int lanes) {
+  if (!h || (lanes != 0 && lanes != 64 && lanes != 128 && lanes != 256))
+    return le2d::set_error(le2d::ERR_ARG, "server: pair window %d", lanes);
+  std::lock_guard<std::mutex> lk(h->s->mu);
+  h->s->pair_fb = lanes;
   h->s->dirty = true;
   return le2d::OK;
 }

@@ -180,8 +180,10 @@ int step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy) {
   if (g->elem != (int)sizeof(T)) return set_error(ERR_ARG, "step2_xy: element size mismatch");
   const int seg = g->tiles.pair_seg;
   if (seg < 1) return set_error(ERR_ARG, "step2_xy: no pair segment height");
-  const int rc = g->tiles.pair_block == 128 ? launch_step2_xy<T, 128>(g, cx, cy, seg)
-                                            : launch_step2_xy<T, 256>(g, cx, cy, seg);
+  const int fb = g->tiles.pair_block;
+  const int rc = fb == 64    ? launch_step2_xy<T, 64>(g, cx, cy, seg)
+                 : fb == 128 ? launch_step2_xy<T, 128>(g, cx, cy, seg)
+                             : launch_step2_xy<T, 256>(g, cx, cy, seg);
   if (rc) return rc;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(ERR_DEVICE, "step2_xy launch: %s", hipGetErrorString(e));

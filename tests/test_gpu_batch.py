@@ -63,7 +63,7 @@ def test_batch_plain_loop_and_shapes_agree():
     for shape, plain in [((128, 32, "tiles"), False), ((64, 7, "persistent"), False),
                          ((256, 1000, "persistent"), False), ((256, 16, "tiles"), False),
                          ((256, 5, "pairs"), False), ((256, 64, "pairs"), False),
-                         ((128, 9, "pairs"), False),
+                         ((128, 9, "pairs"), False), ((64, 11, "pairs"), False),
                          ((128, 32, None), True)]:
         b, states = _make(np.float64)
         with b:

@@ -1,7 +1,7 @@
 """Two steps per pass (le2d_set_step_pairs): bit-identical to single steps for
 every size (one and several strips, grids narrower than a window, one to a
-few rows), segment height (1 row to more than the grid), window width (128
-and 256 lanes) and element type,
+few rows), segment height (1 row to more than the grid), window width (64,
+128 and 256 lanes) and element type,
 odd step counts included (the last step takes the single-step pass), and
 to the oracle in FP64."""
 import numpy as np
@@ -34,7 +34,7 @@ def test_pairs_equal_single_steps(dtype, nx, ny):
     rng = np.random.default_rng(1000 * nx + ny)
     g0 = rng.standard_normal((ny, nx, 5)).astype(dtype)
     single = _run(g0, dtype, 0, 5)
-    for lanes in (256, 128):
+    for lanes in (256, 128, 64):
         for seg in SEGS[1:]:
             assert np.array_equal(_run(g0, dtype, seg, 5, lanes), single), (lanes, seg)
 
@@ -71,7 +71,7 @@ def test_tuning_leaves_the_state_alone():
         seg = t.tune_step_pairs()
         assert seg in (16, 24, 32, 48, 64, 96, 128)
         assert t.get_step_pairs() == seg
-        assert t.get_step_pair_window() in (128, 256)
+        assert t.get_step_pair_window() in (64, 128, 256)
         assert np.array_equal(t.grid(), before)
         t.step_n(9)
         got = t.grid().copy()

@@ -105,9 +105,10 @@ def test_budget_limits_admission_and_rejects_oversize():
 def test_two_step_launches_change_nothing():
     # odd chunk limit and odd step count: two-step launches plus single ones
     res = []
-    for pairs in (0, 64, 5):
+    for pairs, lanes in ((0, 0), (64, 0), (5, 64), (16, 128), (7, 256)):
         with le2d_amd.Server(1 * GB, 16) as s:
             s.set_pairs(pairs)
+            s.set_pair_window(lanes)
             s.set_shape(256, 16, 7)
             ids = [s.submit(**REQ, dtype=dt, keep=True) for dt in (np.float32, np.float64)]
             s.drain(60)
