@@ -42,6 +42,11 @@ struct TaskState {
   int mode = LE2D_SYNC_WAIT;
   hipStream_t stream = nullptr;
   bool pairs_fixed = false;  // segment height set by the caller or tuned
+  // A step accepted by le_step but not yet run, so that it and the next
+  // le_step share one two-step pass; run before anything else touches the
+  // device state (flush_pending), with the coefficients it was called with.
+  bool pending = false;
+  SweepCoef<real> pend_cx, pend_cy;
 };
 
 std::mutex g_mu;
@@ -80,9 +85,11 @@ TaskState *find(const le_task *t) {
   return s;
 }
 
+int flush_pending(TaskState *s);
+
 // Make the device hold the current state (allocate on first use, upload
-// host edits).
-int ensure_device(const le_task *t, TaskState *s) {
+// host edits, run a deferred step unless keep_pending).
+int ensure_device(const le_task *t, TaskState *s, bool keep_pending = false) {
   le2d::require_device();
   if (!s->dev_valid) {
     if (!s->stream) {
@@ -96,17 +103,31 @@ int ensure_device(const le_task *t, TaskState *s) {
     s->host_dirty = true;
   }
   if (s->host_dirty) {
+    s->pending = false;  // the uploaded mirror replaces the device state
     int rc = le2d::grid_upload(&s->g, t->grid, t->stype == ST_SOA);
     if (rc) return rc;
     s->host_dirty = false;
     s->dev_newer = false;
   }
+  return keep_pending ? LE2D_OK : flush_pending(s);
+}
+
+// Run a deferred le_step on its own (single-step pass).
+int flush_pending(TaskState *s) {
+  if (!s->pending) return LE2D_OK;
+  s->pending = false;
+  const int rc = le2d::step_xy<real>(&s->g, s->pend_cx, s->pend_cy, le2d::EDGE_CLAMP);
+  if (rc) return rc;
+  s->dev_newer = true;
   return LE2D_OK;
 }
 
 int sync_host(le_task *t, TaskState *s) {
+  if (s->host_dirty) return LE2D_OK;  // the mirror is the newest state (uploaded next)
+  int rc = flush_pending(s);
+  if (rc) return rc;
   if (!s->dev_valid || !s->dev_newer) return LE2D_OK;
-  int rc = le2d::grid_download(&s->g, t->grid, t->stype == ST_SOA);
+  rc = le2d::grid_download(&s->g, t->grid, t->stype == ST_SOA);
   if (rc) return rc;
   s->dev_newer = false;
   return LE2D_OK;
@@ -210,12 +231,38 @@ bool tuning_enabled() {
   return !(e && !strcmp(e, "0"));
 }
 
-// Queue sweeps on the task's stream, then honour the sync mode.
-int run(le_task *t, int what, int edge, int steps) {
+bool same_coef(const SweepCoef<real> &a, const SweepCoef<real> &b) {
+  return !memcmp(&a, &b, sizeof a);
+}
+
+// Queue sweeps on the task's stream, then honour the sync mode.  defer (one
+// le_step): an odd step of the clamped full step is held back (pending) and
+// runs with the next le_step as one two-step pass -- the same operations, so
+// the same bits -- or alone before any other use of the device state.  Not
+// in the host-coherent mode, whose mirror is current after every call, nor
+// with a material plane.
+int run(le_task *t, int what, int edge, int steps, bool defer = false) {
   TaskState *s = find(t);
-  int rc = ensure_device(t, s);
+  int rc = ensure_device(t, s, true);
   if (rc) return rc;
   const SweepCoef<real> cx = coef(t, t->h.x), cy = coef(t, t->h.y);
+  const bool pairable = what == (DO_X | DO_Y) && edge == le2d::EDGE_CLAMP &&
+                        s->g.tiles.pair_seg > 0 && !s->mat.active;
+  if (s->pending) {
+    // a held step joins this call's steps only with the same coefficients
+    if (pairable && same_coef(s->pend_cx, cx) && same_coef(s->pend_cy, cy)) {
+      s->pending = false;
+      ++steps;
+    } else if ((rc = flush_pending(s))) {
+      return rc;
+    }
+  }
+  if (defer && pairable && s->mode != LE2D_SYNC_COHERENT && (steps & 1)) {
+    s->pending = true;
+    s->pend_cx = cx;
+    s->pend_cy = cy;
+    --steps;
+  }
   if (s->mat.active && what != (DO_X | DO_Y))
     return le2d::set_error(LE2D_ERR_STATE,
                            "single sweeps are not available with a material plane; use le_step");
@@ -250,7 +297,7 @@ int run(le_task *t, int what, int edge, int steps) {
       if (rc) return rc;
     }
   }
-  s->dev_newer = true;
+  if (steps > 0) s->dev_newer = true;
   if (s->mode == LE2D_SYNC_ASYNC) return LE2D_OK;
   hipError_t e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) return le2d::set_error(LE2D_ERR_DEVICE, "step: %s", hipGetErrorString(e));
@@ -258,8 +305,8 @@ int run(le_task *t, int what, int edge, int steps) {
   return LE2D_OK;
 }
 
-void run_or_die(le_task *t, int what, int edge, const char *name) {
-  int rc = run(t, what, edge, 1);
+void run_or_die(le_task *t, int what, int edge, const char *name, bool defer = false) {
+  int rc = run(t, what, edge, 1, defer);
   if (rc) die(name, rc);
 }
 
@@ -427,10 +474,10 @@ void le_step_y(le_task *task) {
   run_or_die(task, DO_Y, le2d::EDGE_CLAMP, "le_step_y");
 }
 
-// simple/le_core.c:605-620.
+// simple/le_core.c:605-620.  Consecutive calls are run in pairs (see run).
 void le_step(le_task *task) {
   assert(task->stype == ST_AOS);
-  run_or_die(task, DO_X | DO_Y, le2d::EDGE_CLAMP, "le_step");
+  run_or_die(task, DO_X | DO_Y, le2d::EDGE_CLAMP, "le_step", true);
 }
 
 // simple/le_core.c:540-602: column blocks of width cfs; the result does not
@@ -476,12 +523,22 @@ const char *le2d_last_error(void) { return le2d::last_error(); }
 int le2d_set_sync_mode(le_task *task, int mode) {
   if (mode < LE2D_SYNC_WAIT || mode > LE2D_SYNC_COHERENT)
     return le2d::set_error(LE2D_ERR_ARG, "sync mode %d", mode);
-  find(task)->mode = mode;
+  TaskState *s = find(task);
+  int rc = flush_pending(s);
+  if (rc) return rc;
+  s->mode = mode;
+  if (mode == LE2D_SYNC_COHERENT && s->stream) {
+    if (hipStreamSynchronize(s->stream) != hipSuccess)
+      return le2d::set_error(LE2D_ERR_DEVICE, "sync mode: stream");
+    return sync_host(task, s);
+  }
   return LE2D_OK;
 }
 
 int le2d_wait(le_task *task) {
   TaskState *s = find(task);
+  int rc = flush_pending(s);
+  if (rc) return rc;
   if (!s->stream) return LE2D_OK;
   hipError_t e = hipStreamSynchronize(s->stream);
   return e == hipSuccess ? LE2D_OK
@@ -810,6 +867,7 @@ int le2d_restore(le_task *task, const char *path) {
   free(buf);
   s->host_dirty = true;
   s->dev_newer = false;
+  s->pending = false;
   return LE2D_OK;
 }
 
