@@ -116,7 +116,7 @@ struct MatPlane {
   // tables of the two-step pass (strips of 248 columns)
   MatTile *d_homog2 = nullptr, *d_mixed2 = nullptr;
   int n_homog2 = 0, n_mixed2 = 0;
-  int built_pair_seg = -1;
+  int built_pair_seg = -1, built_pair_fb = -1;
 };
 
 int mat_set(DevGrid *g, MatPlane *m, int count, const double *c1c2rho, const unsigned char *ids,

@@ -46,23 +46,24 @@ __global__ __launch_bounds__(FB) void k_step_mat_mixed(const T *__restrict__ src
 }
 
 // Two-step pass: single-material tiles run march_pair with their material's
-// constants, the others the node-local two-step march.
-template <typename T>
-__global__ __launch_bounds__(256, 2) void k_step2_mat_homog(const T *__restrict__ src,
-                                                          T *__restrict__ dst, int nx, int ny,
-                                                          long pitch, long plane,
-                                                          const MatTile *__restrict__ tiles,
-                                                          const SweepCoef<T> *__restrict__ cxt,
-                                                          const SweepCoef<T> *__restrict__ cyt) {
-  __shared__ W4<T> sw[256];
-  __shared__ W4<T> sf[256];
+// constants, the others the node-local two-step march; both in windows of FB
+// lanes (strips of FB-8 columns), the grid's two-step window width.
+template <typename T, int FB>
+__global__ __launch_bounds__(FB, 2) void k_step2_mat_homog(const T *__restrict__ src,
+                                                         T *__restrict__ dst, int nx, int ny,
+                                                         long pitch, long plane,
+                                                         const MatTile *__restrict__ tiles,
+                                                         const SweepCoef<T> *__restrict__ cxt,
+                                                         const SweepCoef<T> *__restrict__ cyt) {
+  __shared__ W4<T> sw[FB];
+  __shared__ W4<T> sf[FB];
   const MatTile tj = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const SweepCoef<T> cx = cxt[tj.mat], cy = cyt[tj.mat];
-  march_pair<T, 256>(src, dst, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cx, cy, sw, sf);
+  march_pair<T, FB>(src, dst, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cx, cy, sw, sf);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_step2_mat_mixed(const T *__restrict__ src,
+template <typename T, int FB>
+__global__ __launch_bounds__(FB) void k_step2_mat_mixed(const T *__restrict__ src,
                                                        T *__restrict__ dst, int nx, int ny,
                                                        long pitch, long plane,
                                                        const MatTile *__restrict__ tiles,
@@ -70,10 +71,10 @@ __global__ __launch_bounds__(256) void k_step2_mat_mixed(const T *__restrict__ s
                                                        long idp,
                                                        const SweepCoef<T> *__restrict__ cxt,
                                                        const SweepCoef<T> *__restrict__ cyt) {
-  __shared__ Node<T> sn[256];
+  __shared__ Node<T> sn[FB];
   const MatTile tj = tiles[blockIdx.x];
-  march_pair_mixed<T, 256>(src, dst, ids, idp, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cxt,
-                           cyt, sn);
+  march_pair_mixed<T, FB>(src, dst, ids, idp, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cxt,
+                          cyt, sn);
 }
 
 // Energy density in FP64 (plane strain, isotropic):
@@ -204,7 +205,7 @@ void mat_release(MatPlane *m) {
   m->d_homog = m->d_mixed = m->d_homog2 = m->d_mixed2 = nullptr;
   m->d_etab = nullptr;
   m->n_homog = m->n_mixed = m->n_homog2 = m->n_mixed2 = 0;
-  m->built_fb = m->built_seg = m->built_edge = m->built_pair_seg = -1;
+  m->built_fb = m->built_seg = m->built_edge = m->built_pair_seg = m->built_pair_fb = -1;
 }
 
 // Material and sweep constants in double (le_init_material and the sweeps'
@@ -339,18 +340,39 @@ static int mat_tiles(DevGrid *g, MatPlane *m, int edge) {
   return OK;
 }
 
-// Tables of the two-step pass: 248-column strips read 4 halo columns and 4
-// halo rows on each side.
+static int pair_lanes(const DevGrid *g) {
+  const int fb = g->tiles.pair_block;
+  return fb == 64 || fb == 128 ? fb : 256;
+}
+
+// Tables of the two-step pass: strips of (window - 8) columns read 4 halo
+// columns and 4 halo rows on each side.
 static int mat_tiles2(DevGrid *g, MatPlane *m) {
-  const int seg = g->tiles.pair_seg;
-  if (m->built_pair_seg == seg) return OK;
+  const int seg = g->tiles.pair_seg, fb = pair_lanes(g);
+  if (m->built_pair_seg == seg && m->built_pair_fb == fb) return OK;
   std::vector<MatTile> homog, mixed;
-  split_tiles(m, g->nx, g->ny, 248, 4, seg, 4, EDGE_CLAMP, &homog, &mixed);
+  split_tiles(m, g->nx, g->ny, fb - 8, 4, seg, 4, EDGE_CLAMP, &homog, &mixed);
   int rc = upload_tiles(homog, &m->d_homog2, &m->n_homog2);
   if (!rc) rc = upload_tiles(mixed, &m->d_mixed2, &m->n_mixed2);
   if (rc) return rc;
   m->built_pair_seg = seg;
+  m->built_pair_fb = fb;
   return OK;
+}
+
+template <typename T, int FB>
+static void launch_mat2(DevGrid *g, const MatPlane *m) {
+  const T *src = static_cast<const T *>(g->buf[g->cur]);
+  T *dst = static_cast<T *>(g->buf[g->cur ^ 1]);
+  const SweepCoef<T> *cx = static_cast<const SweepCoef<T> *>(m->d_cx);
+  const SweepCoef<T> *cy = static_cast<const SweepCoef<T> *>(m->d_cy);
+  if (m->n_homog2)
+    hipLaunchKernelGGL((k_step2_mat_homog<T, FB>), dim3(m->n_homog2), dim3(FB), 0, g->stream, src,
+                       dst, g->nx, g->ny, g->pitch, g->plane, m->d_homog2, cx, cy);
+  if (m->n_mixed2)
+    hipLaunchKernelGGL((k_step2_mat_mixed<T, FB>), dim3(m->n_mixed2), dim3(FB), 0, g->stream, src,
+                       dst, g->nx, g->ny, g->pitch, g->plane, m->d_mixed2, m->d_ids, m->idp, cx,
+                       cy);
 }
 
 template <typename T, int EDGE, int FB>
@@ -401,17 +423,13 @@ int step2_xy_mat(DevGrid *g, MatPlane *m) {
   if (g->tiles.pair_seg < 1) return set_error(ERR_ARG, "step2_xy_mat: no pair segment height");
   int rc = mat_tiles2(g, m);
   if (rc) return rc;
-  const T *src = static_cast<const T *>(g->buf[g->cur]);
-  T *dst = static_cast<T *>(g->buf[g->cur ^ 1]);
-  const SweepCoef<T> *cx = static_cast<const SweepCoef<T> *>(m->d_cx);
-  const SweepCoef<T> *cy = static_cast<const SweepCoef<T> *>(m->d_cy);
-  if (m->n_homog2)
-    hipLaunchKernelGGL((k_step2_mat_homog<T>), dim3(m->n_homog2), dim3(256), 0, g->stream, src,
-                       dst, g->nx, g->ny, g->pitch, g->plane, m->d_homog2, cx, cy);
-  if (m->n_mixed2)
-    hipLaunchKernelGGL((k_step2_mat_mixed<T>), dim3(m->n_mixed2), dim3(256), 0, g->stream, src,
-                       dst, g->nx, g->ny, g->pitch, g->plane, m->d_mixed2, m->d_ids, m->idp, cx,
-                       cy);
+  const int fb = pair_lanes(g);
+  if (fb == 64)
+    launch_mat2<T, 64>(g, m);
+  else if (fb == 128)
+    launch_mat2<T, 128>(g, m);
+  else
+    launch_mat2<T, 256>(g, m);
   rc = mcheck(hipGetLastError(), "step2_xy_mat launch");
   if (rc) return rc;
   g->cur ^= 1;

@@ -165,8 +165,9 @@ def test_diff_norms_across_libraries():
 
 @pytest.mark.parametrize("shape,kind", CASES + [((517, 67), "checker"), ((9, 7), "blobs")])
 def test_heterogeneous_pairs_equal_single_steps(shape, kind):
-    # two steps per pass (single-material and node-local mixed tiles) against
-    # one step per pass and the oracle, odd step count
+    # two steps per pass (single-material and node-local mixed tiles, windows
+    # of 256, 128 and 64 lanes) against one step per pass and the oracle, odd
+    # step count
     nx, ny = shape
     rng = np.random.default_rng(7 * nx + ny)
     tab = [(2.0, 1.0, 1.0), (3.1, 1.6, 2.4), (1.4, 0.8, 1.2)]
@@ -177,10 +178,11 @@ def test_heterogeneous_pairs_equal_single_steps(shape, kind):
     g0 = rng.standard_normal((ny, nx, 5))
     dt, h = 0.45 / 3.1, (1.0, 0.9)
     res = []
-    for seg in (0, 1, 6, 64):
+    for seg, lanes in ((0, 256), (1, 256), (6, 128), (64, 256), (3, 64), (64, 64)):
         with _task(g0, dt, h, (2.0, 1.0, 1.0)) as t:
             t.set_materials(tab, ids)
             t.set_step_pairs(seg)
+            t.set_step_pair_window(lanes)
             t.step_n(5)
             res.append(t.grid().copy())
     for r in res[1:]:
