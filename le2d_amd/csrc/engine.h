@@ -90,6 +90,12 @@ int step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy);
 int copy_probe(size_t bytes, int iters, double *ms);
 
 Tiles default_tiles(int elem, int nx, int ny);
+// Workgroups of the two-step kernel (element size elem, fb lanes) resident on
+// the current device at once (occupancy API x compute units); 0 if unknown.
+long pair_resident(int elem, int fb);
+// Segment height of the two-step pass for an nx x ny grid in windows of fb
+// lanes, from the occupancy model in default_tiles.
+int pair_seg_for(int elem, int nx, int ny, int fb);
 // One grid row (five padded field rows) fits the 31-bit range of a buffer
 // resource, which the row loads and stores address through.
 bool row_fits(int elem, int nx);

@@ -1,5 +1,6 @@
 // engine.hip -- device memory, host<->device transfers and error state of
 // the sweep engine.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -219,28 +220,47 @@ Tiles default_tiles(int elem, int nx, int ny) {
   if (seg < 32) seg = 32;
   if (seg > ny) seg = ny;
   t.y_seg = (int)seg;
-  // two-step pass (le_step_n), measured on MI355X (Gnode-updates/s):
-  // one-wave windows (64 lanes, 56 columns: no barriers between waves) in
-  // the tallest of 64/32/16-row segments that still gives ~4096 workgroups
-  // (FP64 8192^2 87.8 at 64 rows against 85.1 at 128 lanes and 84.5 at 256;
-  // 8192 x 1024 71.8 at 32 rows against 65.7; 2048^2 59.8 at 16 against
-  // 54.8).  FP32, issue-bound rather than latency-bound, keeps 256-lane
-  // windows (248 columns, 3% halo) where 64-row segments of them still give
-  // ~4096 workgroups (8192^2: 229.5 against 206.8 with 64 lanes).
-  {
-    const long strips256 = (nx + 247) / 248;
-    if (elem == 4 && (long)ny * strips256 >= 64L * 4096) {
-      t.pair_block = 256;
-      t.pair_seg = 64;
-    } else {
-      t.pair_block = 64;
-      const long strips = (nx + 55) / 56;
-      int seg = 64;
-      while (seg > 16 && (long)((ny + seg - 1) / seg) * strips < 4096) seg /= 2;
-      t.pair_seg = seg;
+  // two-step pass (le_step_n, paired le_step calls), measured on MI355X:
+  // FP64 takes one-wave windows (64 lanes, 56 columns, no barriers between
+  // waves; 8192^2 at 80-row segments 90 Gnode-updates/s against 85 with 128
+  // or 256 lanes).  FP32, issue- rather than latency-bound, keeps the 3% halo
+  // of 256-lane windows on grids 4096 or more columns wide (8192^2 229
+  // against 210; 2048^2 137 against 153 with 64 lanes).
+  t.pair_block = elem == 8 || nx < 4096 ? 64 : 256;
+  t.pair_seg = pair_seg_for(elem, nx, ny, t.pair_block);
+  return t;
+}
+
+// Segment height of the two-step pass: the candidate height that maximises
+// (fill of the last round of resident workgroups) x (share of a segment's
+// rows that are output rather than the 8 re-read halo rows).  The grid's
+// workgroups run in rounds of `resident`; a last round that is mostly empty
+// leaves the device idle.  Taller segments than 80 rows (64-lane windows)
+// or 64 (wider) measured slower than the model says and are not taken.
+// Measured optima it reproduces (Gnode-updates/s): FP64 64 lanes, 80 rows
+// on 8192^2 (90.2; 86.4 at 160), 8192 x 1024 (75.2; 66.7 at 32) and 4096^2
+// (79.7), 48 on 2048^2 (61.0); FP32 256 lanes, 64 rows on 8192^2 (229.4;
+// 223.5 at 128) and 16384^2 (231.1; 223.2 at 160).
+int pair_seg_for(int elem, int nx, int ny, int fb) {
+  long resident = pair_resident(elem, fb);
+  if (resident <= 0) resident = 2048;
+  const long strips = (nx + fb - 9) / (fb - 8);
+  const int cap = fb == 64 ? 80 : 64;
+  int best = 16;
+  double best_score = -1;
+  for (int s : {16, 24, 32, 48, 64, 80}) {
+    if (s > cap || (s > ny && s != 16)) break;
+    const long blocks = strips * ((ny + s - 1) / s);
+    const long rounds = (blocks + resident - 1) / resident;
+    const double fill = (double)blocks / (double)(rounds * resident);
+    const double rows = std::min(s, ny);
+    const double score = fill * rows / (rows + 8);
+    if (score > best_score + 1e-9) {
+      best_score = score;
+      best = s;
     }
   }
-  return t;
+  return best;
 }
 
 // One 16-byte vector per lane, one pass: the fastest copy shape measured on

@@ -172,7 +172,7 @@ int tune_pairs(le_task *t, TaskState *s) {
     }
   }
   const SweepCoef<real> cx = coef(t, t->h.x), cy = coef(t, t->h.y);
-  const int cand[] = {16, 24, 32, 48, 64, 96, 128};
+  const int cand[] = {16, 24, 32, 48, 64, 80, 96, 128, 160};
   hipEvent_t ev[2];
   for (auto &e : ev) (void)hipEventCreate(&e);
   int rc = LE2D_OK;

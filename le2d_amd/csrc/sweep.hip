@@ -1,5 +1,9 @@
 // sweep.hip -- the x-sweep, y-sweep and fused-step kernels for gfx950 and
 // their launchers.  Numerics and the fused-step tile march: sweep_dev.h.
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "sweep_dev.h"
 
 namespace le2d {
@@ -189,6 +193,37 @@ int step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy) {
   if (e != hipSuccess) return set_error(ERR_DEVICE, "step2_xy launch: %s", hipGetErrorString(e));
   g->cur ^= 1;
   return OK;
+}
+
+template <typename T, int FB>
+static int occupancy_of() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step2_xy<T, FB>, FB, 0) != hipSuccess)
+    return 0;
+  return n;
+}
+
+long pair_resident(int elem, int fb) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, long> cache;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  std::lock_guard<std::mutex> lk(mu);
+  const auto key = std::make_tuple(dev, elem, fb);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per = 0;
+  if (elem == 8)
+    per = fb == 64 ? occupancy_of<double, 64>()
+                   : (fb == 128 ? occupancy_of<double, 128>() : occupancy_of<double, 256>());
+  else
+    per = fb == 64 ? occupancy_of<float, 64>()
+                   : (fb == 128 ? occupancy_of<float, 128>() : occupancy_of<float, 256>());
+  const long r = (long)per * cus;
+  cache[key] = r;
+  return r;
 }
 
 template int step2_xy<double>(DevGrid *, const SweepCoef<double> &, const SweepCoef<double> &);
