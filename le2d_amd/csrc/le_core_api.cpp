@@ -155,10 +155,10 @@ enum { DO_X = 1, DO_Y = 2 };
 // Two-step segment height and window width for this grid: each candidate is
 // timed out of place -- the pass reads the current state and writes the idle
 // buffer (scratch between steps), and the buffer swap is undone -- so the
-// state does not advance.  Every candidate is timed over 4 passes, the best
-// three again over 12 (single short timings of neighbouring candidates differ
-// by less than their noise); the fastest is kept, and remembered for the grid
-// shape.
+// state does not advance.  After a warm-up, every candidate is timed over 4
+// passes, the best three again over 12 (single short timings of neighbouring
+// candidates differ by less than their noise); the fastest is kept, and
+// remembered for the grid shape.
 int tune_pairs(le_task *t, TaskState *s) {
   const auto key = std::make_tuple((int)sizeof(real), s->g.nx, s->g.ny);
   {
@@ -198,6 +198,16 @@ int tune_pairs(le_task *t, TaskState *s) {
     float ms;
     int block, seg;
   };
+  // warm-up at the default shape, about 25 ms: after host-side work the
+  // device clocks are still low and would make the first candidates look slow
+  {
+    const int b0 = s->g.tiles.pair_block, s0 = s->g.tiles.pair_seg;
+    const float est = time_one(b0, s0, 1);
+    if (est > 0) {
+      const int reps = std::max(1, std::min(64, (int)(25.0f / est)));
+      (void)time_one(b0, s0, reps);
+    }
+  }
   std::vector<Cand> res;
   for (int block : {256, 128, 64})
     for (int seg : cand) {
