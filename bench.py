@@ -497,7 +497,7 @@ def main():
                                                     args.warmup, args.profile_steps)
     value = n * n * args.steps / el / 1e9
     roof = roofline(n, es, ms_pair, ms_step, ms_x, ms_y, args.dtype)
-    tiles = t.get_tiles() + (t.get_step_pairs(),)
+    tiles = t.get_tiles() + (t.get_step_pairs(), t.get_step_pair_window())
     other = None
     if not args.no_fp32:
         o_dt = np.float32 if main_dt == np.float64 else np.float64
@@ -545,7 +545,7 @@ def main():
                                f"({cx:.0f},{cy:.0f}), c1=2 c2=1 rho=1, dt=0.25",
                    "nx": n, "ny": n,
                    "tiles": dict(zip(("x_rows", "y_seg", "y_block", "xy_seg", "xy_block",
-                                      "pair_seg"), tiles))},
+                                      "pair_seg", "pair_lanes"), tiles))},
         "hbm_copy_GBps": round(copy_gbs, 1),
         "roofline": roof,
         "cpu_baseline": cpu,
