@@ -68,8 +68,8 @@ template <typename T, int FB>
 __global__ __launch_bounds__(FB, 2) void k_step_batch_pairs(const InstDesc<T> *__restrict__ inst,
                                                           const TileJob *__restrict__ tiles,
                                                           int step_parity) {
-  __shared__ W4<T> sw[FB];
-  __shared__ W4<T> sf[FB];
+  __shared__ W4<T> sw[pair_lds<FB>()];
+  __shared__ W4<T> sf[pair_lds<FB>()];
   const TileJob tj = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const InstDesc<T> &d = inst[tj.inst];
   const int p = d.par ^ step_parity;

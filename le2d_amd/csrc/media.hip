@@ -55,8 +55,8 @@ __global__ __launch_bounds__(FB, 2) void k_step2_mat_homog(const T *__restrict__
                                                          const MatTile *__restrict__ tiles,
                                                          const SweepCoef<T> *__restrict__ cxt,
                                                          const SweepCoef<T> *__restrict__ cyt) {
-  __shared__ W4<T> sw[FB];
-  __shared__ W4<T> sf[FB];
+  __shared__ W4<T> sw[pair_lds<FB>()];
+  __shared__ W4<T> sf[pair_lds<FB>()];
   const MatTile tj = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const SweepCoef<T> cx = cxt[tj.mat], cy = cyt[tj.mat];
   march_pair<T, FB>(src, dst, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cx, cy, sw, sf);

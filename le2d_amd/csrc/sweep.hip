@@ -121,8 +121,8 @@ template <typename T, int FB>
 __global__ __launch_bounds__(FB, 2) void k_step2_xy(const T *__restrict__ src, T *__restrict__ dst,
                                                  int nx, int ny, long pitch, long plane, int seg,
                                                  int nstrips, SweepCoef<T> cx, SweepCoef<T> cy) {
-  __shared__ W4<T> sw[FB];
-  __shared__ W4<T> sf[FB];
+  __shared__ W4<T> sw[pair_lds<FB>()];
+  __shared__ W4<T> sf[pair_lds<FB>()];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int strip = tile % nstrips, segi = tile / nstrips;
   const int j0 = segi * seg;
