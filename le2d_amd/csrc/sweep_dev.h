@@ -610,7 +610,7 @@ __device__ __forceinline__ void march_tile_mixed(const T *__restrict__ src, T *_
     Node<T> u = load_row_node(src + (long)row * pitch, plane, boff);
     const int m = ids[(long)row * idp + xc];
     sn[t] = u;
-    __syncthreads();
+    window_sync<FB>();
     if (inner) {
       const SweepCoef<T> c = cxt[m];
       W4<T> w[5];
@@ -623,7 +623,7 @@ __device__ __forceinline__ void march_tile_mixed(const T *__restrict__ src, T *_
       const W4<T> fr = faces(w[1], w[2], w[3], w[4], c);
       apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w[2], fl, fr, c);
     }
-    __syncthreads();
+    window_sync<FB>();
     return u;
   };
 
@@ -678,7 +678,7 @@ __device__ __forceinline__ void march_pair_mixed(const T *__restrict__ src, T *_
   auto xsweep = [&](Node<T> u, int row) -> Node<T> {
     const int m = ids[(long)row * idp + xc];
     sn[t] = u;
-    __syncthreads();
+    window_sync<FB>();
     if (inner) {
       const SweepCoef<T> c = cxt[m];
       W4<T> w[5];
@@ -691,7 +691,7 @@ __device__ __forceinline__ void march_pair_mixed(const T *__restrict__ src, T *_
       const W4<T> fr = faces(w[1], w[2], w[3], w[4], c);
       apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w[2], fl, fr, c);
     }
-    __syncthreads();
+    window_sync<FB>();
     return u;
   };
   // node-local y-update of row j from the x-swept rows j-2 .. j+2
