@@ -203,9 +203,12 @@ Tiles default_tiles(int elem, int nx, int ny) {
   {
     // fused step: 256-lane windows (252 output columns) and ~64 workgroups
     // per CU in segments of at least 16 rows (measured best on 8192^2: 16-24
-    // rows; short segments keep the halo rows of neighbouring segments in L2)
-    t.xy_block = 256;
-    const long strips = (nx + 251) / 252;
+    // rows; short segments keep the halo rows of neighbouring segments in L2).
+    // Grids under 4096 columns take one-wave windows (60 columns, no
+    // barriers between waves): 2048^2 FP64 50.5 against 45.9 Gnode-updates/s,
+    // FP32 144 against 137.
+    t.xy_block = nx >= 4096 ? 256 : 64;
+    const long strips = (nx + t.xy_block - 5) / (t.xy_block - 4);
     const long want = (16384 + strips - 1) / strips;
     long seg = (ny + want - 1) / want;
     if (seg < 16) seg = 16;
