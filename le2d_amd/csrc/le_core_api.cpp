@@ -241,6 +241,15 @@ bool tuning_enabled() {
   return !(e && !strcmp(e, "0"));
 }
 
+// LE2D_PAIR_CALLS=0: every le_step runs its own step before returning.
+bool pairing_enabled() {
+  static const bool on = [] {
+    const char *e = getenv("LE2D_PAIR_CALLS");
+    return !(e && !strcmp(e, "0"));
+  }();
+  return on;
+}
+
 bool same_coef(const SweepCoef<real> &a, const SweepCoef<real> &b) {
   return !memcmp(&a, &b, sizeof a);
 }
@@ -267,7 +276,7 @@ int run(le_task *t, int what, int edge, int steps, bool defer = false) {
       return rc;
     }
   }
-  if (defer && pairable && s->mode != LE2D_SYNC_COHERENT && (steps & 1)) {
+  if (defer && pairable && s->mode != LE2D_SYNC_COHERENT && (steps & 1) && pairing_enabled()) {
     s->pending = true;
     s->pend_cx = cx;
     s->pend_cy = cy;
