@@ -11,7 +11,9 @@ import le2d_amd
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
-SIZES = [(3, 3), (4, 5), (7, 3), (16, 9), (33, 20), (250, 7), (300, 257), (517, 129), (1031, 67)]
+SIZES = [(3, 3), (4, 5), (7, 3), (16, 9), (33, 20), (250, 7), (300, 257), (517, 129), (1031, 67),
+         # widths at and one past the strip widths of 64-, 128- and 256-lane windows
+         (56, 9), (57, 11), (112, 6), (120, 5), (121, 7), (248, 4), (249, 5), (496, 3)]
 SEGS = (0, 1, 3, 7, 32, 1000)
 ARGS = (0.2, (1.0, 0.9), (2.0, 1.0, 1.5))
 
