@@ -76,8 +76,9 @@ int le2d_get_step_pairs(le_task *task, int *seg);
  * of about 8192^2 and more. */
 int le2d_set_step_pair_window(le_task *task, int lanes);
 int le2d_get_step_pair_window(le_task *task, int *lanes);
-/* Time the two-step pass at segment heights 16 .. 160 and windows of 64, 128
- * and 256 lanes on the task's grid and keep the fastest.  The state does not
+/* Time the two-step pass at segment heights 16 .. 160 (4 .. 160 on grids too
+ * small to fill the device with 16-row segments) and windows of 64, 128 and
+ * 256 lanes on the task's grid and keep the fastest.  The state does not
  * advance: each candidate reads the current state and writes the idle
  * buffer.  le_step_n does this once per
  * grid shape and process on its first call of 8 or more steps, unless the

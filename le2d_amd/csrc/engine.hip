@@ -243,16 +243,19 @@ Tiles default_tiles(int elem, int nx, int ny) {
 // Measured optima it reproduces (Gnode-updates/s): FP64 64 lanes, 80 rows
 // on 8192^2 (90.2; 86.4 at 160), 8192 x 1024 (75.2; 66.7 at 32) and 4096^2
 // (79.7), 48 on 2048^2 (61.0); FP32 256 lanes, 64 rows on 8192^2 (229.4;
-// 223.5 at 128) and 16384^2 (231.1; 223.2 at 160).
+// 223.5 at 128) and 16384^2 (231.1; 223.2 at 160).  On grids whose
+// workgroups all fit in one round the pass lasts one workgroup's march, so
+// short segments win: FP64 12 rows on 1024^2 (43.2; 36.6 at 16), 4 on 512^2
+// (18.2; 13.4 at 16), 2 on 256^2 (7.3; 3.5 at 16).
 int pair_seg_for(int elem, int nx, int ny, int fb) {
   long resident = pair_resident(elem, fb);
   if (resident <= 0) resident = 2048;
   const long strips = (nx + fb - 9) / (fb - 8);
   const int cap = fb == 64 ? 80 : 64;
-  int best = 16;
+  int best = 2;
   double best_score = -1;
-  for (int s : {16, 24, 32, 48, 64, 80}) {
-    if (s > cap || (s > ny && s != 16)) break;
+  for (int s : {2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 80}) {
+    if (s > cap || (s > ny && s != 2)) break;
     const long blocks = strips * ((ny + s - 1) / s);
     const long rounds = (blocks + resident - 1) / resident;
     const double fill = (double)blocks / (double)(rounds * resident);

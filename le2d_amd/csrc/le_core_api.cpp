@@ -172,7 +172,7 @@ int tune_pairs(le_task *t, TaskState *s) {
     }
   }
   const SweepCoef<real> cx = coef(t, t->h.x), cy = coef(t, t->h.y);
-  const int cand[] = {16, 24, 32, 48, 64, 80, 96, 128, 160};
+  const int cand[] = {4, 8, 12, 16, 24, 32, 48, 64, 80, 96, 128, 160};
   hipEvent_t ev[2];
   for (auto &e : ev) (void)hipEventCreate(&e);
   int rc = LE2D_OK;
@@ -209,13 +209,21 @@ int tune_pairs(le_task *t, TaskState *s) {
     }
   }
   std::vector<Cand> res;
-  for (int block : {256, 128, 64})
+  for (int block : {256, 128, 64}) {
+    // segments under 16 rows only for grids that 16-row segments leave short
+    // of one round of resident workgroups (there a pass lasts one
+    // workgroup's march)
+    const long strips = (s->g.nx + block - 9) / (block - 8);
+    const long resident = le2d::pair_resident((int)sizeof(real), block);
+    const bool small = strips * ((s->g.ny + 15) / 16) < (resident > 0 ? resident : 2048);
     for (int seg : cand) {
+      if (seg < 16 && !small) continue;
       if (seg > s->g.ny && seg != cand[0]) break;  // taller segments are all one
       const float ms = time_one(block, seg, 4);
       if (ms < 0) break;
       res.push_back({ms, block, seg});
     }
+  }
   Cand best{1e30f, 256, 64};
   if (!rc && !res.empty()) {
     std::sort(res.begin(), res.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });

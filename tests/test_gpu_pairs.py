@@ -71,7 +71,7 @@ def test_tuning_leaves_the_state_alone():
         t.step_n(3)
         before = t.grid().copy()
         seg = t.tune_step_pairs()
-        assert seg in (16, 24, 32, 48, 64, 80, 96, 128, 160)
+        assert seg in (4, 8, 12, 16, 24, 32, 48, 64, 80, 96, 128, 160)
         assert t.get_step_pairs() == seg
         assert t.get_step_pair_window() in (64, 128, 256)
         assert np.array_equal(t.grid(), before)
