@@ -93,6 +93,8 @@ Tiles default_tiles(int elem, int nx, int ny);
 // Workgroups of the two-step kernel (element size elem, fb lanes) resident on
 // the current device at once (occupancy API x compute units); 0 if unknown.
 long pair_resident(int elem, int fb);
+// The same for the single-step kernel (clamped edges).
+long step_resident(int elem, int fb);
 // Segment height of the two-step pass for an nx x ny grid in windows of fb
 // lanes, from the occupancy model in default_tiles.
 int pair_seg_for(int elem, int nx, int ny, int fb);

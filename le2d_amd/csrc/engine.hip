@@ -197,6 +197,8 @@ int grid_download(DevGrid *g, void *host, int soa) {
   return check(hipStreamSynchronize(g->stream), "download sync");
 }
 
+static int best_segment(long strips, int ny, long resident, int halo, int cap);
+
 Tiles default_tiles(int elem, int nx, int ny) {
   Tiles t;
   t.x_rows = 4;
@@ -214,6 +216,15 @@ Tiles default_tiles(int elem, int nx, int ny) {
     if (seg < 16) seg = 16;
     if (seg > ny) seg = ny;
     t.xy_seg = (int)seg;
+    // grids that 16-row segments leave short of one round of resident
+    // workgroups: a step lasts one workgroup's march (4 halo rows, then the
+    // segment), so the occupancy model of the two-step pass picks a shorter
+    // segment (512^2 FP64 16.5 Gnode-updates/s at 2-4 rows against 11.5 at
+    // 16; 1024^2 39.6 at 12 against 33.9)
+    long resident = step_resident(elem, t.xy_block);
+    if (resident <= 0) resident = 768;
+    if (strips * ((ny + 15) / 16) < resident)
+      t.xy_seg = best_segment(strips, ny, resident, 4, 16);
   }
   t.y_block = nx >= 8 * 256 ? 256 : (nx >= 8 * 128 ? 128 : 64);
   // enough segments for ~8 workgroups per CU, but at least 32 rows each
@@ -234,6 +245,27 @@ Tiles default_tiles(int elem, int nx, int ny) {
   return t;
 }
 
+// The candidate segment height (2 .. cap rows) that maximises (fill of the
+// last round of `resident` workgroups) x (share of a segment's rows that are
+// output rather than the `halo` re-read rows).
+static int best_segment(long strips, int ny, long resident, int halo, int cap) {
+  int best = 2;
+  double best_score = -1;
+  for (int s : {2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 80}) {
+    if (s > cap || (s > ny && s != 2)) break;
+    const long blocks = strips * ((ny + s - 1) / s);
+    const long rounds = (blocks + resident - 1) / resident;
+    const double fill = (double)blocks / (double)(rounds * resident);
+    const double rows = std::min(s, ny);
+    const double score = fill * rows / (rows + halo);
+    if (score > best_score + 1e-9) {
+      best_score = score;
+      best = s;
+    }
+  }
+  return best;
+}
+
 // Segment height of the two-step pass: the candidate height that maximises
 // (fill of the last round of resident workgroups) x (share of a segment's
 // rows that are output rather than the 8 re-read halo rows).  The grid's
@@ -251,22 +283,7 @@ int pair_seg_for(int elem, int nx, int ny, int fb) {
   long resident = pair_resident(elem, fb);
   if (resident <= 0) resident = 2048;
   const long strips = (nx + fb - 9) / (fb - 8);
-  const int cap = fb == 64 ? 80 : 64;
-  int best = 2;
-  double best_score = -1;
-  for (int s : {2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 80}) {
-    if (s > cap || (s > ny && s != 2)) break;
-    const long blocks = strips * ((ny + s - 1) / s);
-    const long rounds = (blocks + resident - 1) / resident;
-    const double fill = (double)blocks / (double)(rounds * resident);
-    const double rows = std::min(s, ny);
-    const double score = fill * rows / (rows + 8);
-    if (score > best_score + 1e-9) {
-      best_score = score;
-      best = s;
-    }
-  }
-  return best;
+  return best_segment(strips, ny, resident, 8, fb == 64 ? 80 : 64);
 }
 
 // One 16-byte vector per lane, one pass: the fastest copy shape measured on

@@ -195,15 +195,20 @@ int step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T> &cy) {
   return OK;
 }
 
-template <typename T, int FB>
+template <typename T, int FB, bool PAIR>
 static int occupancy_of() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step2_xy<T, FB>, FB, 0) != hipSuccess)
-    return 0;
-  return n;
+  hipError_t e;
+  if constexpr (PAIR)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step2_xy<T, FB>, FB, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_step_xy<T, EDGE_CLAMP, FB>, FB, 0);
+  return e == hipSuccess ? n : 0;
 }
 
-long pair_resident(int elem, int fb) {
+// Workgroups per CU x CUs of the current device, cached per device.
+template <bool PAIR>
+static long resident_of(int elem, int fb) {
   static std::mutex mu;
   static std::map<std::tuple<int, int, int>, long> cache;
   int dev = 0, cus = 0;
@@ -216,15 +221,20 @@ long pair_resident(int elem, int fb) {
   if (it != cache.end()) return it->second;
   int per = 0;
   if (elem == 8)
-    per = fb == 64 ? occupancy_of<double, 64>()
-                   : (fb == 128 ? occupancy_of<double, 128>() : occupancy_of<double, 256>());
+    per = fb == 64    ? occupancy_of<double, 64, PAIR>()
+          : fb == 128 ? occupancy_of<double, 128, PAIR>()
+                      : occupancy_of<double, 256, PAIR>();
   else
-    per = fb == 64 ? occupancy_of<float, 64>()
-                   : (fb == 128 ? occupancy_of<float, 128>() : occupancy_of<float, 256>());
+    per = fb == 64    ? occupancy_of<float, 64, PAIR>()
+          : fb == 128 ? occupancy_of<float, 128, PAIR>()
+                      : occupancy_of<float, 256, PAIR>();
   const long r = (long)per * cus;
   cache[key] = r;
   return r;
 }
+
+long pair_resident(int elem, int fb) { return resident_of<true>(elem, fb); }
+long step_resident(int elem, int fb) { return resident_of<false>(elem, fb); }
 
 template int step2_xy<double>(DevGrid *, const SweepCoef<double> &, const SweepCoef<double> &);
 template int step2_xy<float>(DevGrid *, const SweepCoef<float> &, const SweepCoef<float> &);
