@@ -121,10 +121,14 @@ struct MatPlane {
   MatTile *d_homog = nullptr, *d_mixed = nullptr;
   int n_homog = 0, n_mixed = 0;
   int built_fb = -1, built_seg = -1, built_edge = -1;
-  // tables of the two-step pass (strips of 248 columns)
+  // tables of the two-step pass (strips of the two-step window less 8 columns)
   MatTile *d_homog2 = nullptr, *d_mixed2 = nullptr;
   int n_homog2 = 0, n_mixed2 = 0;
   int built_pair_seg = -1, built_pair_fb = -1;
+  // the interface tiles run on a second stream beside the single-material
+  // ones (forked from and joined back into the grid's stream)
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 int mat_set(DevGrid *g, MatPlane *m, int count, const double *c1c2rho, const unsigned char *ids,
