@@ -235,8 +235,9 @@ le2d_server *le2d_server_create(size_t hbm_budget, int max_live, int plain);
 /* Launch shape of the batched steps and the longest run of steps between
  * admission points. */
 int le2d_server_set_shape(le2d_server *srv, int fb, int seg, int max_chunk);
-/* Rows per segment of the two-step launches (default 64); 0: one step per
- * launch.  Results do not change. */
+/* Largest rows per segment of the two-step launches (default 64; the server
+ * picks the height for its live set, shorter when few requests are live);
+ * 0: one step per launch.  Results do not change. */
 int le2d_server_set_pairs(le2d_server *srv, int pair_seg);
 /* Lanes per two-step workgroup, 64, 128 or 256; 0 (the default): 256 for
  * FP32 requests and 64 for FP64.  Results do not change. */

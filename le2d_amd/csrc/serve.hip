@@ -105,7 +105,7 @@ struct Server {
   int plain;
   int max_chunk = 16;
   int fb = 256, seg = 16;
-  int pair_seg = 64;  // two-step launches (segment rows); 0: one step per launch
+  int pair_seg = 64;  // two-step launches: largest segment height; 0: one step per launch
   // lanes per two-step workgroup (64, 128 or 256); 0: 256 for FP32 requests,
   // which are issue-bound, 64 (one wavefront, no barriers between waves)
   // for FP64, which are latency-bound
@@ -201,10 +201,39 @@ struct Server {
     return OK;
   }
 
+  // Two-step segment height for the live set of one precision: the height,
+  // at most pair_seg rows, that maximises (fill of the last round of
+  // resident workgroups) x (share of rows that are output rather than halo)
+  // over all live requests.  Many live requests keep tall segments; the few
+  // of a light load or of the tail of a burst take short ones, as a small
+  // grid does.
+  int live_pair_seg(int si, const std::vector<Req *> &set) const {
+    const int lanes = pair_lanes(si);
+    long resident = pair_resident(si == 1 ? 8 : 4, lanes);
+    if (resident <= 0) resident = 2048;
+    int best = std::min(pair_seg, 2);
+    double best_score = -1;
+    for (int s : {2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 80, 96, 128}) {
+      if (s > pair_seg) break;
+      long blocks = 0;
+      for (const Req *r : set)
+        blocks += (long)((r->spec.nx + lanes - 9) / (lanes - 8)) * ((r->spec.ny + s - 1) / s);
+      if (blocks <= 0) break;
+      const long rounds = (blocks + resident - 1) / resident;
+      const double score = (double)blocks / (double)(rounds * resident) * s / (s + 8);
+      if (score > best_score + 1e-9) {
+        best_score = score;
+        best = s;
+      }
+    }
+    return best;
+  }
+
   template <typename T>
   int build_tables(int si, const std::vector<Req *> &set) {
     std::vector<InstDesc<T>> desc(set.size());
     std::vector<TileJob> tiles, tiles2;
+    const int seg2 = pair_seg > 0 ? live_pair_seg(si, set) : 0;
     for (size_t i = 0; i < set.size(); ++i) {
       const Req &r = *set[i];
       const le2d_request &q = r.spec;
@@ -225,7 +254,7 @@ struct Server {
       if (pair_seg > 0) {  // strips of lanes - 8 output columns (k_step_batch_pairs)
         const int lanes = pair_lanes(si);
         const int strips2 = (q.nx + lanes - 9) / (lanes - 8);
-        const int sg2 = std::max(1, std::min(pair_seg, q.ny));
+        const int sg2 = std::max(1, std::min(seg2, q.ny));
         for (int j0 = 0; j0 < q.ny; j0 += sg2)
           for (int s = 0; s < strips2; ++s)
             tiles2.push_back({(int)i, s, j0, std::min(j0 + sg2, q.ny), j0 / sg2});
