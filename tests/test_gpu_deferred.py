@@ -107,3 +107,28 @@ def test_coherent_mode_and_device_reads():
         t.step()                                 # no deferral: mirror current
         O.run(ref, *ARGS, 1)
         assert _bits(t.view(), ref)
+
+
+def test_pairing_off_by_environment(tmp_path):
+    # LE2D_PAIR_CALLS=0 (read once per process): every le_step runs its own
+    # step; checked in a child process against the oracle
+    import os
+    import subprocess
+    import sys
+    g0 = _g0(8)
+    src, out = tmp_path / "g0.npy", tmp_path / "got.npy"
+    np.save(src, g0)
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, %r); import le2d_amd\n"
+        "g0 = np.load(%r)\n"
+        "with le2d_amd.Task(%d, %d, %r, %r, %r) as t:\n"
+        "    t.load(g0)\n"
+        "    for _ in range(3): t.step()\n"
+        "    np.save(%r, t.grid())\n"
+    ) % (os.path.dirname(os.path.dirname(os.path.abspath(__file__))), str(src), NX, NY,
+         ARGS[0], ARGS[1], ARGS[2], str(out))
+    env = dict(os.environ, LE2D_PAIR_CALLS="0")
+    subprocess.run([sys.executable, "-c", code], env=env, check=True, timeout=120)
+    ref = g0.copy()
+    O.run(ref, *ARGS, 3)
+    assert _bits(np.load(out), ref)
