@@ -128,6 +128,10 @@ struct Server {
   TileJob *d_tiles2[2] = {nullptr, nullptr};  // two-step table
   size_t cap_inst[2] = {0, 0}, cap_tiles[2] = {0, 0}, cap_tiles2[2] = {0, 0};
   int ntiles[2] = {0, 0}, ntiles2[2] = {0, 0};
+  // the requests of the current tables, by steps left (most first), and the
+  // number of single-step / two-step tiles of the first k of them
+  std::vector<Req *> order[2];
+  std::vector<int> pre1[2], pre2[2];
   bool dirty = true;
   // statistics
   long long completed = 0;
@@ -234,6 +238,8 @@ struct Server {
     std::vector<InstDesc<T>> desc(set.size());
     std::vector<TileJob> tiles, tiles2;
     const int seg2 = pair_seg > 0 ? live_pair_seg(si, set) : 0;
+    pre1[si].assign(1, 0);
+    pre2[si].assign(1, 0);
     for (size_t i = 0; i < set.size(); ++i) {
       const Req &r = *set[i];
       const le2d_request &q = r.spec;
@@ -259,6 +265,8 @@ struct Server {
           for (int s = 0; s < strips2; ++s)
             tiles2.push_back({(int)i, s, j0, std::min(j0 + sg2, q.ny), j0 / sg2});
       }
+      pre1[si].push_back((int)tiles.size());
+      pre2[si].push_back((int)tiles2.size());
     }
     if (desc.size() * sizeof(InstDesc<T>) > cap_inst[si]) {
       if (d_inst[si]) (void)hipFree(d_inst[si]);
@@ -280,27 +288,29 @@ struct Server {
     return OK;
   }
 
+  // One step over single-step tiles [first, last) of one precision.
   template <typename T>
-  void launch_batch(int si, int step_parity) {
+  void launch_batch(int si, int step_parity, int first, int last) {
+    if (last <= first) return;
     const InstDesc<T> *d = static_cast<const InstDesc<T> *>(d_inst[si]);
-    if (ntiles[si] <= 0) return;
-    const dim3 grid((unsigned)ntiles[si]);
+    const dim3 grid((unsigned)(last - first));
+    const TileJob *tj = d_tiles[si] + first;
     if (fb == 64)
-      hipLaunchKernelGGL((k_step_batch_tiles<T, 64>), grid, dim3(64), 0, stream[si], d,
-                         d_tiles[si], step_parity);
+      hipLaunchKernelGGL((k_step_batch_tiles<T, 64>), grid, dim3(64), 0, stream[si], d, tj,
+                         step_parity);
     else if (fb == 128)
-      hipLaunchKernelGGL((k_step_batch_tiles<T, 128>), grid, dim3(128), 0, stream[si], d,
-                         d_tiles[si], step_parity);
+      hipLaunchKernelGGL((k_step_batch_tiles<T, 128>), grid, dim3(128), 0, stream[si], d, tj,
+                         step_parity);
     else
-      hipLaunchKernelGGL((k_step_batch_tiles<T, 256>), grid, dim3(256), 0, stream[si], d,
-                         d_tiles[si], step_parity);
+      hipLaunchKernelGGL((k_step_batch_tiles<T, 256>), grid, dim3(256), 0, stream[si], d, tj,
+                         step_parity);
   }
 
-  // Two steps of every live request of one precision.
+  // Two steps over the first `count` two-step tiles of one precision.
   template <typename T>
-  void launch_pairs(int si, int step_parity) {
-    if (ntiles2[si] <= 0) return;
-    const dim3 grid((unsigned)ntiles2[si]);
+  void launch_pairs(int si, int step_parity, int count) {
+    if (count <= 0) return;
+    const dim3 grid((unsigned)count);
     const InstDesc<T> *d = static_cast<const InstDesc<T> *>(d_inst[si]);
     const int lanes = pair_lanes(si);
     if (lanes == 64)
@@ -312,6 +322,39 @@ struct Server {
     else
       hipLaunchKernelGGL((k_step_batch_pairs<T, 256>), grid, dim3(256), 0, stream[si], d,
                          d_tiles2[si], step_parity);
+  }
+
+  // Requests of the current tables of precision si with at least k steps
+  // this chunk (the tables are ordered by steps left, most first).
+  int count_at_least(int si, int k, int chunk) const {
+    int n = 0;
+    for (const Req *r : order[si])
+      if (std::min(r->left, chunk) >= k) ++n;
+      else break;
+    return n;
+  }
+
+  // The launches of one chunk.  Each live request takes min(left, chunk)
+  // steps: two-step launch p covers the requests with at least 2p+2 steps
+  // this chunk (a prefix of the tables), and a single-step launch beside it
+  // the requests with exactly 2p+1 (the next run of them).  Every request
+  // in launch p has taken exactly p launches before it in this chunk, so
+  // its buffer parity is its table parity xor (p & 1).  Without two-step
+  // launches, launch s covers the requests with at least s+1 steps.
+  template <typename T>
+  void launch_chunk(int si, int chunk) {
+    if (order[si].empty()) return;
+    if (pair_seg > 0) {
+      for (int p = 0; 2 * p < chunk; ++p) {
+        const int n2 = 2 * p + 2 <= chunk ? count_at_least(si, 2 * p + 2, chunk) : 0;
+        const int n1 = count_at_least(si, 2 * p + 1, chunk);
+        launch_pairs<T>(si, p & 1, pre2[si][n2]);
+        launch_batch<T>(si, p & 1, pre1[si][n2], pre1[si][n1]);
+      }
+    } else {
+      for (int s = 0; s < chunk; ++s)
+        launch_batch<T>(si, s & 1, 0, pre1[si][count_at_least(si, s + 1, chunk)]);
+    }
   }
 
   // The serial baseline's launch on the one live request: the single-grid
@@ -358,16 +401,20 @@ struct Server {
           if (!live.empty()) break;
           cv_work.wait(lk);
         }
-        chunk = max_chunk;
+        // requests that finish inside a chunk stop there (launch_chunk), so
+        // a chunk runs max_chunk steps (even, with two-step launches) unless
+        // every live request has fewer left
+        int most = 0;
         for (long long id : live) {
           Req &r = reqs[id];
-          chunk = std::min(chunk, r.left);
+          most = std::max(most, r.left);
           set[r.spec.real_size == 8 ? 1 : 0].push_back(&r);
         }
+        chunk = std::min(max_chunk, most);
+        if (!plain && pair_seg > 0 && chunk >= 2) chunk &= ~1;
       }
       const double t0 = now_s();
       int rc = OK;
-      int launches = 0;
       if (plain) {
         Req &r = set[0].empty() ? *set[1][0] : *set[0][0];
         for (int s = 0; s < chunk && !rc;) {
@@ -378,21 +425,18 @@ struct Server {
         }
       } else {
         if (dirty) {
+          for (int si = 0; si < 2; ++si) {
+            std::stable_sort(set[si].begin(), set[si].end(),
+                             [](const Req *a, const Req *b) { return a->left > b->left; });
+            order[si] = set[si];
+          }
           if (!set[0].empty()) rc = build_tables<float>(0, set[0]);
           if (!rc && !set[1].empty()) rc = build_tables<double>(1, set[1]);
           dirty = false;
         }
-        // two steps per launch while two remain, then one; each launch flips
-        // the buffer parity
-        int s = 0;
-        if (pair_seg > 0)
-          for (; s + 2 <= chunk && !rc; s += 2, ++launches) {
-            if (!set[0].empty()) launch_pairs<float>(0, launches & 1);
-            if (!set[1].empty()) launch_pairs<double>(1, launches & 1);
-          }
-        for (; s < chunk && !rc; ++s, ++launches) {
-          if (!set[0].empty()) launch_batch<float>(0, launches & 1);
-          if (!set[1].empty()) launch_batch<double>(1, launches & 1);
+        if (!rc) {
+          launch_chunk<float>(0, chunk);
+          launch_chunk<double>(1, chunk);
         }
         if (!rc && hipGetLastError() != hipSuccess) rc = set_error(ERR_DEVICE, "server: launch");
       }
@@ -406,11 +450,17 @@ struct Server {
       std::lock_guard<std::mutex> lk(mu);
       busy_s += t1 - t0;
       std::vector<long long> still;
+      bool stale = false;  // a table parity no longer names the current buffer
       for (long long id : live) {
         Req &r = reqs[id];
-        if (!plain) r.cur ^= launches & 1;
-        r.left -= chunk;
-        node_updates += (double)r.spec.nx * r.spec.ny * chunk;
+        const int taken = std::min(r.left, chunk);
+        if (!plain) {
+          const int n = pair_seg > 0 ? taken / 2 + (taken & 1) : taken;  // launches taken
+          r.cur ^= n & 1;
+          stale |= (n & 1) && r.left > taken;
+        }
+        r.left -= taken;
+        node_updates += (double)r.spec.nx * r.spec.ny * taken;
         if (r.left == 0) {
           r.state = DONE;
           r.t_done = t1;
@@ -429,11 +479,9 @@ struct Server {
         dirty = true;
         cv_done.notify_all();
       }
-      if (!plain) {
-        // tables hold each request's parity at build time: rebuild after an
-        // odd number of launches so par stays the current buffer
-        if (launches & 1) dirty = true;
-      }
+      // tables hold each request's parity at build time: rebuild after an
+      // odd number of launches for any request that stays
+      if (stale) dirty = true;
     }
   }
 };
