@@ -475,7 +475,8 @@ class Server:
         self._check(self.lib.le2d_server_set_shape(self.h, fb, seg, max_chunk), "set_shape")
 
     def set_pairs(self, pair_seg=64):
-        """Rows per segment of the two-step launches; 0: one step per launch."""
+        """Largest rows per segment of the two-step launches (the server picks
+        the height for its live set); 0: one step per launch."""
         self._check(self.lib.le2d_server_set_pairs(self.h, pair_seg), "set_pairs")
 
     def set_pair_window(self, lanes=0):
