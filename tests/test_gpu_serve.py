@@ -116,3 +116,29 @@ def test_two_step_launches_change_nothing():
     for other in res[1:]:
         for a, b in zip(res[0], other):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("chunk", [8, 5])
+def test_every_request_of_a_ragged_mix_matches_oracle(chunk):
+    # requests that finish inside a chunk (odd and even step counts, one
+    # step, steps beyond several chunks) each take exactly their own steps:
+    # every kept result is bit-identical to the oracle
+    rng = np.random.default_rng(31 + chunk)
+    specs = []
+    for steps in (1, 2, 3, 7, 8, 9, 13, 16, 17, 30, 41):
+        nx, ny = (int(rng.integers(24, 160)) for _ in range(2))
+        c1 = float(rng.uniform(1, 4))
+        specs.append(dict(nx=nx, ny=ny, steps=steps, dt=0.45 / c1, h=(1.0, 1.0),
+                          mat=(c1, c1 / float(rng.uniform(1.5, 2.2)), float(rng.uniform(1, 3))),
+                          disk=(nx / 2.0, ny / 3.0, float(rng.uniform(3, 9)), 1.0)))
+    with le2d_amd.Server(1 * GB, 64) as s:
+        s.set_shape(64, 16, chunk)
+        ids = [s.submit(**q, dtype=np.float64, keep=True) for q in specs]
+        s.drain(60)
+        got = [s.result(i) for i in ids]
+    for q, g in zip(specs, got):
+        ref = np.zeros((q["ny"], q["nx"], 5))
+        cx, cy, r, sv = q["disk"]
+        OD.set_ball(ref, q["h"], (cx, cy), r, sv)
+        OD.run(ref, q["dt"], q["h"], q["mat"], q["steps"])
+        assert np.array_equal(g, ref), (q["nx"], q["ny"], q["steps"])
