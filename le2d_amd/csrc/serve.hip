@@ -110,7 +110,14 @@ struct Server {
   // which are issue-bound, 64 (one wavefront, no barriers between waves)
   // for FP64, which are latency-bound
   int pair_fb = 0;
-  int pair_lanes(int si) const { return pair_fb ? pair_fb : (si == 1 ? 64 : 256); }
+  // The settings above are written by API calls (under mu); the engine
+  // thread launches with this copy, taken under mu whenever it rebuilds the
+  // tables, so a setting that changes mid-chunk never meets tables built
+  // for another.
+  struct Launch {
+    int fb = 256, seg = 16, pair_seg = 64, pair_fb = 0;
+  } act;
+  int pair_lanes(int si) const { return act.pair_fb ? act.pair_fb : (si == 1 ? 64 : 256); }
   char *arena_dev = nullptr;
   Arena arena;
   std::mutex mu;
@@ -215,10 +222,10 @@ struct Server {
     const int lanes = pair_lanes(si);
     long resident = pair_resident(si == 1 ? 8 : 4, lanes);
     if (resident <= 0) resident = 2048;
-    int best = std::min(pair_seg, 2);
+    int best = std::min(act.pair_seg, 2);
     double best_score = -1;
     for (int s : {2, 4, 6, 8, 12, 16, 24, 32, 48, 64, 80, 96, 128}) {
-      if (s > pair_seg) break;
+      if (s > act.pair_seg) break;
       long blocks = 0;
       for (const Req *r : set)
         blocks += (long)((r->spec.nx + lanes - 9) / (lanes - 8)) * ((r->spec.ny + s - 1) / s);
@@ -237,7 +244,7 @@ struct Server {
   int build_tables(int si, const std::vector<Req *> &set) {
     std::vector<InstDesc<T>> desc(set.size());
     std::vector<TileJob> tiles, tiles2;
-    const int seg2 = pair_seg > 0 ? live_pair_seg(si, set) : 0;
+    const int seg2 = act.pair_seg > 0 ? live_pair_seg(si, set) : 0;
     pre1[si].assign(1, 0);
     pre2[si].assign(1, 0);
     for (size_t i = 0; i < set.size(); ++i) {
@@ -252,12 +259,12 @@ struct Server {
       d.plane = r.plane;
       d.par = r.cur;
       coef_from_material<T>(q.c1, q.c2, q.rho, q.dt, q.hx, q.hy, &d.cx, &d.cy);
-      const int strips = (q.nx + fb - 5) / (fb - 4);
-      const int sg = std::max(1, std::min(seg, q.ny));
+      const int strips = (q.nx + act.fb - 5) / (act.fb - 4);
+      const int sg = std::max(1, std::min(act.seg, q.ny));
       for (int j0 = 0; j0 < q.ny; j0 += sg)
         for (int s = 0; s < strips; ++s)
           tiles.push_back({(int)i, s, j0, std::min(j0 + sg, q.ny), j0 / sg});
-      if (pair_seg > 0) {  // strips of lanes - 8 output columns (k_step_batch_pairs)
+      if (act.pair_seg > 0) {  // strips of lanes - 8 output columns (k_step_batch_pairs)
         const int lanes = pair_lanes(si);
         const int strips2 = (q.nx + lanes - 9) / (lanes - 8);
         const int sg2 = std::max(1, std::min(seg2, q.ny));
@@ -295,10 +302,10 @@ struct Server {
     const InstDesc<T> *d = static_cast<const InstDesc<T> *>(d_inst[si]);
     const dim3 grid((unsigned)(last - first));
     const TileJob *tj = d_tiles[si] + first;
-    if (fb == 64)
+    if (act.fb == 64)
       hipLaunchKernelGGL((k_step_batch_tiles<T, 64>), grid, dim3(64), 0, stream[si], d, tj,
                          step_parity);
-    else if (fb == 128)
+    else if (act.fb == 128)
       hipLaunchKernelGGL((k_step_batch_tiles<T, 128>), grid, dim3(128), 0, stream[si], d, tj,
                          step_parity);
     else
@@ -344,7 +351,7 @@ struct Server {
   template <typename T>
   void launch_chunk(int si, int chunk) {
     if (order[si].empty()) return;
-    if (pair_seg > 0) {
+    if (act.pair_seg > 0) {
       for (int p = 0; 2 * p < chunk; ++p) {
         const int n2 = 2 * p + 2 <= chunk ? count_at_least(si, 2 * p + 2, chunk) : 0;
         const int n1 = count_at_least(si, 2 * p + 1, chunk);
@@ -390,6 +397,7 @@ struct Server {
     for (;;) {
       std::vector<Req *> set[2];
       int chunk = 0;
+      bool rebuild = false;
       {
         std::unique_lock<std::mutex> lk(mu);
         // Admission runs on every wake: a submit (or a released result) that
@@ -410,8 +418,13 @@ struct Server {
           most = std::max(most, r.left);
           set[r.spec.real_size == 8 ? 1 : 0].push_back(&r);
         }
+        if (dirty) {
+          act = {fb, seg, pair_seg, pair_fb};
+          rebuild = true;
+          dirty = false;
+        }
         chunk = std::min(max_chunk, most);
-        if (!plain && pair_seg > 0 && chunk >= 2) chunk &= ~1;
+        if (!plain && act.pair_seg > 0 && chunk >= 2) chunk &= ~1;
       }
       const double t0 = now_s();
       int rc = OK;
@@ -424,7 +437,7 @@ struct Server {
           s += two ? 2 : 1;
         }
       } else {
-        if (dirty) {
+        if (rebuild) {
           for (int si = 0; si < 2; ++si) {
             std::stable_sort(set[si].begin(), set[si].end(),
                              [](const Req *a, const Req *b) { return a->left > b->left; });
@@ -432,7 +445,6 @@ struct Server {
           }
           if (!set[0].empty()) rc = build_tables<float>(0, set[0]);
           if (!rc && !set[1].empty()) rc = build_tables<double>(1, set[1]);
-          dirty = false;
         }
         if (!rc) {
           launch_chunk<float>(0, chunk);
@@ -455,7 +467,7 @@ struct Server {
         Req &r = reqs[id];
         const int taken = std::min(r.left, chunk);
         if (!plain) {
-          const int n = pair_seg > 0 ? taken / 2 + (taken & 1) : taken;  // launches taken
+          const int n = act.pair_seg > 0 ? taken / 2 + (taken & 1) : taken;  // launches taken
           r.cur ^= n & 1;
           stale |= (n & 1) && r.left > taken;
         }
