@@ -142,3 +142,35 @@ def test_every_request_of_a_ragged_mix_matches_oracle(chunk):
         OD.set_ball(ref, q["h"], (cx, cy), r, sv)
         OD.run(ref, q["dt"], q["h"], q["mat"], q["steps"])
         assert np.array_equal(g, ref), (q["nx"], q["ny"], q["steps"])
+
+
+def test_settings_changed_while_running():
+    # launch settings changed from another thread while requests run take
+    # effect at a table rebuild; results stay bit-identical to the oracle
+    rng = np.random.default_rng(77)
+    specs = []
+    for steps in (11, 24, 37, 50):
+        nx, ny = (int(rng.integers(64, 200)) for _ in range(2))
+        specs.append(dict(nx=nx, ny=ny, steps=steps, dt=0.2, h=(1.0, 1.0), mat=(2.0, 1.0, 1.5),
+                          disk=(nx / 2.0, ny / 2.0, 6.0, 1.0)))
+    with le2d_amd.Server(1 * GB, 16) as s:
+        s.set_shape(64, 16, 4)
+        ids = [s.submit(**q, dtype=np.float64, keep=True) for q in specs]
+
+        def toggle():
+            for k in range(40):
+                s.set_pair_window((64, 128, 256, 0)[k % 4])
+                s.set_pairs((8, 64, 3, 16)[k % 4])
+                s.set_shape((64, 128, 256)[k % 3], 16, 4)
+                time.sleep(0.0005)
+        th = threading.Thread(target=toggle)
+        th.start()
+        th.join()
+        s.drain(60)
+        got = [s.result(i) for i in ids]
+    for q, g in zip(specs, got):
+        ref = np.zeros((q["ny"], q["nx"], 5))
+        cx, cy, r, sv = q["disk"]
+        OD.set_ball(ref, q["h"], (cx, cy), r, sv)
+        OD.run(ref, q["dt"], q["h"], q["mat"], q["steps"])
+        assert np.array_equal(g, ref), q["steps"]
