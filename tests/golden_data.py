@@ -40,3 +40,13 @@ def fromhex(x):
 def sha256(a):
     import hashlib
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def same_bits(a, b):
+    """Equal shapes, dtypes and bit patterns, signed zeros included
+    (np.array_equal treats -0 and +0 as equal)."""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape or a.dtype != b.dtype:
+        return False
+    u = {8: np.uint64, 4: np.uint32}[a.dtype.itemsize]
+    return bool(np.array_equal(a.view(u), b.view(u)))

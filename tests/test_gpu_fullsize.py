@@ -7,6 +7,7 @@ within its stated tolerance of FP64."""
 import numpy as np
 import pytest
 
+import golden_data as gd
 import le2d_amd
 from oracle import Oracle
 
@@ -28,7 +29,7 @@ def test_fullsize_fp64_bit_identical_to_oracle():
         t.step_n(3)
         got = t.grid()
         Oracle(np.float64).run(ref, 0.25, (1.0, 1.0), MAT, 3)
-        assert np.array_equal(got, ref)
+        assert gd.same_bits(got, ref)
 
 
 def test_fullsize_field_sums_conserved():
