@@ -382,6 +382,16 @@ def pct(v, q):
     return float(np.percentile(np.asarray(v), q)) if len(v) else None
 
 
+def drain_logged(s, n, what, limit_s):
+    """Drain the server, printing how many of its n requests are done every
+    20 s; raises after limit_s seconds."""
+    t0 = time.perf_counter()
+    while not s.drained(20.0):
+        if time.perf_counter() - t0 > limit_s:
+            raise RuntimeError(f"{what}: not drained after {limit_s:.0f} s")
+        log(f"{what}: {s.stats()['completed']} of {n} requests done")
+
+
 def serve_report(le2d_amd, window_s=1.0, budget_gb=200.0, slots=1024):
     reqs = serve_requests(window_s)
     work = sum(q["nx"] * q["ny"] * q["steps"] for q in reqs)
@@ -411,7 +421,7 @@ def serve_report(le2d_amd, window_s=1.0, budget_gb=200.0, slots=1024):
     log(f"serving {len(reqs)} requests")
     with le2d_amd.Server(budget, slots) as s:
         ids = submit_all(s, True, keep=probe)
-        s.drain(300)
+        drain_logged(s, len(reqs), "batched", 300 + 10 * window_s)
         times = [s.times(i) for i in ids]
         st = s.stats()
         mixed = {k: s.result(ids[k]) for k in probe}
@@ -442,7 +452,7 @@ def serve_report(le2d_amd, window_s=1.0, budget_gb=200.0, slots=1024):
     log("serial baseline")
     with le2d_amd.Server(budget, 1, plain=True) as s:
         ids = submit_all(s, False)
-        s.drain(300)
+        drain_logged(s, len(reqs), "serial", 300 + 10 * window_s)
         # the engine admits request k+1 when k completes: its service time
         # (initialisation and steps) is the gap between completions
         done = [s.times(i)[2] for i in ids]

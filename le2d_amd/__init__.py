@@ -502,6 +502,16 @@ class Server:
     def drain(self, timeout=-1.0):
         self._check(self.lib.le2d_server_drain(self.h, timeout), "le2d_server_drain")
 
+    def drained(self, timeout):
+        """Wait up to `timeout` seconds for every submitted request: True when
+        all are done, False when the time ran out (LE2D_ERR_STATE); any
+        other failure raises."""
+        rc = self.lib.le2d_server_drain(self.h, timeout)
+        if rc == 4:
+            return False
+        self._check(rc, "le2d_server_drain")
+        return True
+
     def result(self, i):
         shape, dt = self.shapes[i]
         a = np.empty(shape + (5,), dtype=dt)
