@@ -175,3 +175,15 @@ def test_settings_changed_while_running():
         OD.set_ball(ref, q["h"], (cx, cy), r, sv)
         OD.run(ref, q["dt"], q["h"], q["mat"], q["steps"])
         assert gd.same_bits(g, ref), q["steps"]
+
+
+def test_drained_reports_timeout_then_completion():
+    with le2d_amd.Server(1 * GB, 8) as s:
+        ids = [s.submit(**REQ, dtype=np.float64, keep=True) for _ in range(3)]
+        first = s.drained(0.0)  # a zero wait returns at once, done or not
+        assert first in (True, False)
+        assert s.drained(60.0)
+        assert s.stats()["completed"] == 3
+        for i in ids:
+            ts, ta, td = s.times(i)
+            assert ts <= ta <= td
