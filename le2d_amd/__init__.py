@@ -18,6 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(HERE, "lib")
 
 SYNC_WAIT, SYNC_ASYNC, SYNC_COHERENT = 0, 1, 2
+# LE2D_ERR_* status codes of include/le2d.h
+ERR_ARG, ERR_DEVICE, ERR_NOMEM, ERR_STATE, ERR_BUDGET = 1, 2, 3, 4, 5
 
 # Symbols declared in include/le2d.h (extensions), beyond simple/le_core.h.
 _EXT = {
@@ -507,7 +509,7 @@ class Server:
         all are done, False when the time ran out (LE2D_ERR_STATE); any
         other failure raises."""
         rc = self.lib.le2d_server_drain(self.h, timeout)
-        if rc == 4:
+        if rc == ERR_STATE:
             return False
         self._check(rc, "le2d_server_drain")
         return True

@@ -48,6 +48,16 @@ def test_headers_declare_the_reference_api():
         assert n in names
 
 
+def test_python_constants_match_header():
+    text = open(os.path.join(ROOT, "include", "le2d.h")).read()
+    defs = {m.group(1): int(m.group(2))
+            for m in re.finditer(r"^#define LE2D_([A-Z_]+) (\d+)\s*$", text, flags=re.M)}
+    for name in ("ERR_ARG", "ERR_DEVICE", "ERR_NOMEM", "ERR_STATE", "ERR_BUDGET"):
+        assert getattr(le2d_amd, name) == defs[name], name
+    for name in ("SYNC_WAIT", "SYNC_ASYNC", "SYNC_COHERENT"):
+        assert getattr(le2d_amd, name) == defs[name], name
+
+
 def test_every_declared_symbol_is_exported(lib):
     out = subprocess.run(["nm", "-D", "--defined-only", lib._name], capture_output=True,
                          text=True, check=True).stdout
