@@ -257,18 +257,27 @@ def mixed_instances(count=256, seed=3):
     return out
 
 
-def run_batch(le2d_amd, dtype, insts, steps, plain_steps):
-    """Batched launch (autotuned, graph-replayed) vs one plain launch per
-    instance per step, same instances."""
+def run_batch(le2d_amd, dtype, insts, steps, plain_steps, shape=None):
+    """Batched launch (autotuned, or the two-step form at a given (window,
+    segment) shape; graph-replayed) vs one plain launch per instance per
+    step, same instances."""
+    name = np.dtype(dtype).name
     b = le2d_amd.Batch(dtype)
     for p in insts:
         b.add(p["nx"], p["ny"], p["dt"], (1.0, 1.0), (p["c1"], p["c2"], p["rho"]))
     b.commit()
-    fb, seg, _ = b.autotune(2)  # on the zero state, which stays exactly zero
+    if shape is None:
+        log(f"batch {name}: autotune")
+        fb, seg, _ = b.autotune(2)  # on the zero state, which stays exactly zero
+    else:
+        fb, seg = shape
+        b.set_shape(fb, seg, "pairs")
     for i, p in enumerate(insts):
         b.set_ball(i, (p["cx"], p["cy"]), p["r"], 1.0)
     nodes = sum(p["nx"] * p["ny"] for p in insts)
+    log(f"batch {name}: {steps} batched steps at {fb} lanes x {seg} rows")
     ms_batch = b.time(steps)
+    log(f"batch {name}: {plain_steps} steps one launch per instance")
     ms_plain = b.time(plain_steps, plain=True)
     b.wait()
     es = np.dtype(dtype).itemsize
@@ -286,12 +295,12 @@ def run_batch(le2d_amd, dtype, insts, steps, plain_steps):
     return res, b
 
 
-def batch_report(le2d_amd, steps=500, plain_steps=50, count=256):
+def batch_report(le2d_amd, steps=500, plain_steps=50, count=256, shape=None):
     insts = mixed_instances(count)
     out = {"workload": f"{count} instances, sides log-uniform [128, 2048], per-instance material "
                        f"and disk, {steps} steps timed batched, {plain_steps} per-instance"}
-    r64, b64 = run_batch(le2d_amd, np.float64, insts, steps, plain_steps)
-    r32, b32 = run_batch(le2d_amd, np.float32, insts, steps, plain_steps)
+    r64, b64 = run_batch(le2d_amd, np.float64, insts, steps, plain_steps, shape)
+    r32, b32 = run_batch(le2d_amd, np.float32, insts, steps, plain_steps, shape)
     # both advanced the same number of steps from the same disks
     e2 = r2 = 0.0
     einf = rinf = worst = 0.0
@@ -489,6 +498,11 @@ def main():
     ap.add_argument("--serve-window", type=float, default=1.0,
                     help="seconds of Poisson arrivals in the serving leg")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--batch-plain-steps", type=int, default=50,
+                    help="steps of the one-launch-per-instance comparison loop")
+    ap.add_argument("--batch-shape", default=None, metavar="LANES,ROWS",
+                    help="two-step batch launches at this window and segment height "
+                         "instead of autotuning (the same kernel in every profiling pass)")
     ap.add_argument("--tier-steps", type=int, default=2000)
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="launches timed per kernel with HIP events for the roofline")
@@ -530,7 +544,9 @@ def main():
     batched = None
     if not args.no_batch:
         log("mixed-size batch")
-        batched = batch_report(le2d_amd, steps=args.batch_steps)
+        shape = tuple(int(v) for v in args.batch_shape.split(",")) if args.batch_shape else None
+        batched = batch_report(le2d_amd, steps=args.batch_steps,
+                               plain_steps=args.batch_plain_steps, shape=shape)
     ref_case = None if args.no_ref else reference_case_report(le2d_amd)
     host = None if args.no_host else host_boundary_report(le2d_amd, n)
     slab = None if args.no_slab else slab_report(le2d_amd, n)
