@@ -205,6 +205,27 @@ def test_x_sweep_is_four_scalar_advections():
     assert np.array_equal(got[0], got[1]) and np.array_equal(got[1], got[2])
 
 
+@pytest.mark.parametrize("mat", [(2.0, 1.0, 1.5), (3.1, 1.6, 2.3)])
+def test_sweeps_keep_their_null_space_combination(mat):
+    """An x-sweep adds (e2-e1)*rhoc1 to sxx and (e2-e1)*rhoc3 to syy, so
+    syy - (rhoc3/rhoc1)*sxx does not change; a y-sweep keeps
+    sxx - (rhoc3/rhoc1)*syy.  A swapped rhoc1/rhoc3 or stress field in the
+    increments breaks this."""
+    m = OD.material(*mat)
+    q = m.rhoc3 / m.rhoc1
+    rng = np.random.default_rng(9)
+    g = rng.standard_normal((20, 24, 5))  # (vx, vy, sxx, sxy, syy)
+    x_keep = lambda a: a[..., 4] - q * a[..., 2]  # noqa: E731
+    y_keep = lambda a: a[..., 2] - q * a[..., 4]  # noqa: E731
+    for sweep, keep, other in ((OD.sweep_x, x_keep, y_keep), (OD.sweep_y, y_keep, x_keep)):
+        a = g.copy()
+        sweep(a, 0.2, 1.0, mat)
+        np.testing.assert_allclose(keep(a), keep(g), rtol=0, atol=1e-12)
+        # the stresses did change, and the other sweep's combination with them
+        assert np.abs(a[..., 2] - g[..., 2]).max() > 1e-2
+        assert np.abs(other(a) - other(g)).max() > 1e-2
+
+
 def test_conservation_while_edges_are_quiet():
     g = np.zeros((96, 96, 5))
     OD.set_ball(g, (1.0, 1.0), (48.0, 48.0), 8.0, 1.0)
