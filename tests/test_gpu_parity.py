@@ -218,3 +218,36 @@ def test_set_ball_after_steps_sees_current_state():
     OD.set_ball(ref, (1.0, 1.0), (10.0, 50.0), 4.0, -1.0)
     OD.run(ref, 0.25, (1.0, 1.0), (2.0, 1.0, 1.0), 1)
     assert gd.same_bits(got, ref)
+
+
+def test_tasks_stepped_from_two_threads():
+    """Distinct tasks stepped concurrently from two host threads (one stream
+    per task; the segment tuner and its cache shared) each stay
+    bit-identical to the oracle."""
+    import threading
+    specs = [((300, 257), 0.2, (1.0, 1.0), (2.0, 1.0, 1.5), 21),
+             ((1024, 96), 0.15, (0.9, 1.1), (2.6, 1.4, 1.2), 17)]
+    rng = np.random.default_rng(16)
+    inits = [rng.standard_normal((ny, nx, 5)) for (nx, ny), *_ in specs]
+    out, errs = [None, None], []
+
+    def work(k):
+        try:
+            (nx, ny), dt, h, mat, steps = specs[k]
+            with make(inits[k], dt, h, mat) as t:
+                t.step_n(steps - 5)  # long enough for the segment tuner
+                for _ in range(5):
+                    t.step()  # calls paired into two-step passes
+                out[k] = t.grid().copy()
+        except Exception as e:  # reported by the main thread
+            errs.append(e)
+    th = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join(120)
+    assert not errs, errs
+    for k, ((nx, ny), dt, h, mat, steps) in enumerate(specs):
+        ref = inits[k].copy()
+        OD.run(ref, dt, h, mat, steps)
+        assert gd.same_bits(out[k], ref), specs[k][0]
