@@ -57,8 +57,13 @@ def run_plain(le2d_amd, n, dtype, steps, warmup, profile_steps):
     """Times `steps` steps through le_step_n (two steps per pass); then times
     the two-step kernel, the single-step kernel and the two sweep kernels per
     launch with HIP events.  Every run of this function advances the task by
-    warmup + steps + 4*profile_steps steps."""
+    warmup + steps + 4*profile_steps steps.  The two-step segment is tuned
+    first (the state does not advance), so that no warm-up count leaves the
+    tuning that le_step_n would do on its first call of 8 or more steps
+    inside the timed region (LE2D_TUNE=0 keeps the built-in choice)."""
     t = plain_task(le2d_amd, n, dtype)
+    if os.environ.get("LE2D_TUNE") != "0" and t.get_step_pairs() > 0:
+        t.tune_step_pairs()
     t.step_n(max(1, warmup))
     t.wait()
     t0 = time.perf_counter()
