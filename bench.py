@@ -280,6 +280,8 @@ def run_batch(le2d_amd, dtype, insts, steps, plain_steps, shape=None):
     for i, p in enumerate(insts):
         b.set_ball(i, (p["cx"], p["cy"]), p["r"], 1.0)
     nodes = sum(p["nx"] * p["ny"] for p in insts)
+    b.step(4)  # warm-up: the step graph (two launches) is captured here, outside the timing
+    b.wait()
     log(f"batch {name}: {steps} batched steps at {fb} lanes x {seg} rows")
     ms_batch = b.time(steps)
     log(f"batch {name}: {plain_steps} steps one launch per instance")
