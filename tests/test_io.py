@@ -44,6 +44,31 @@ def test_dump_restore_roundtrip_host(tmp_path, dtype, stype):
                 c.restore(p)
 
 
+def test_restore_rejects_bad_files_and_keeps_the_task(tmp_path):
+    rng = np.random.default_rng(4)
+    with le2d_amd.Task(17, 9, 0.2, dtype=np.float32) as a:
+        a.load(rng.standard_normal(a.view().shape).astype(np.float32))
+        good = str(tmp_path / "f.raw")
+        a.dump(good)
+    raw = open(good, "rb").read()
+    bad = {"magic": b"XXXXRAW1" + raw[8:], "truncated": raw[:-4], "header": raw[:100]}
+    for name, data in bad.items():
+        with open(tmp_path / name, "wb") as f:
+            f.write(data)
+    with le2d_amd.Task(17, 9, 0.3, dtype=np.float32) as b:
+        keep = rng.standard_normal(b.view().shape).astype(np.float32)
+        b.load(keep)
+        for name in bad:
+            with pytest.raises(le2d_amd.LE2DError):
+                b.restore(str(tmp_path / name))
+        with pytest.raises(le2d_amd.LE2DError):
+            b.restore(str(tmp_path / "missing.raw"))
+        assert np.array_equal(b.view(), keep) and b.t.dt == np.float32(0.3)
+    with le2d_amd.Task(17, 9, 0.3, dtype=np.float64) as c:  # FP32 file, FP64 task
+        with pytest.raises(le2d_amd.LE2DError):
+            c.restore(good)
+
+
 def _run(exe, args, cwd, env=None):
     e = dict(os.environ)
     e.update(env or {})
