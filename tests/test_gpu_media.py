@@ -191,3 +191,27 @@ def test_heterogeneous_pairs_equal_single_steps(shape, kind):
     ref = g0.copy()
     OD.run_mat(ref, dt, h, tab, ids, 5)
     assert gd.same_bits(res[0], ref)
+
+
+def test_heterogeneous_long_trajectory_matches_oracle():
+    """40 steps, three materials (a midline split and a blob), waves from two
+    disks crossing both interfaces; default launch shapes (two steps per
+    pass), bit-identical to the oracle."""
+    nx, ny = 517, 331
+    y, x = np.mgrid[0:ny, 0:nx]
+    ids = np.zeros((ny, nx), np.uint8)
+    ids[:, nx // 2:] = 1
+    ids[((x - 150) ** 2 + (y - 200) ** 2) < 60 ** 2] = 2
+    tab = [(2.0, 1.0, 1.0), (3.1, 1.6, 2.4), (1.4, 0.8, 1.2)]
+    dt, h = 0.45 / 3.1, (1.0, 1.0)
+    g0 = np.zeros((ny, nx, 5))
+    OD.set_ball(g0, h, (240.0, 170.0), 30.0, 1.0)
+    OD.set_ball(g0, h, (150.0, 200.0), 12.0, -0.5)
+    ref = g0.copy()
+    OD.run_mat(ref, dt, h, tab, ids, 40)
+    # the waves reached both interfaces
+    assert np.abs(ref[:, nx // 2]).max() > 1e-3 and np.abs(ref[200, 150 + 60]).max() > 1e-3
+    with _task(g0, dt, h, (2.0, 1.0, 1.0)) as t:
+        t.set_materials(tab, ids)
+        t.step_n(40)
+        assert gd.same_bits(t.grid(), ref)
