@@ -133,3 +133,24 @@ def test_instance_row_too_long_is_rejected():
         assert b.add(1000, 3, 0.25) == 0
     finally:
         b.close()
+
+
+def test_batch_long_trajectory_matches_oracle():
+    """31 steps (graph replays of two-step launches, then one single step)
+    from a disk in every instance: each instance bit-identical to the oracle
+    stepping it alone."""
+    b = le2d_amd.Batch(np.float64)
+    inits = []
+    for k, (nx, ny) in enumerate(SIZES):
+        dt, h, mat = _params(k)
+        b.add(nx, ny, dt, h, mat)
+        g = np.zeros((ny, nx, 5))
+        OD.set_ball(g, h, (0.5 * nx * h[0], 0.4 * ny * h[1]), 0.3 * max(nx * h[0], ny * h[1]), 1.0)
+        inits.append(g)
+    b.commit()
+    with b:
+        for k, g in enumerate(inits):
+            b.load(k, g)
+        b.step(31)
+        for k, r in enumerate(_oracle(inits, 31)):
+            assert gd.same_bits(b.read(k), r), SIZES[k]
