@@ -66,14 +66,29 @@ def run_plain(le2d_amd, n, dtype, steps, warmup, profile_steps):
         t.tune_step_pairs()
     t.step_n(max(1, warmup))
     t.wait()
-    t0 = time.perf_counter()
-    t.step_n(steps)
-    t.wait()
-    el = time.perf_counter() - t0
+    els = time_runs(t, steps, 1)
     ms_pair = t.time_pairs(profile_steps) if t.get_step_pairs() > 0 else None
     ms_step = t.time_steps(profile_steps)
     ms_x, ms_y = t.time_sweeps(profile_steps)
-    return t, el, ms_pair, ms_step, ms_x, ms_y
+    return t, els, ms_pair, ms_step, ms_x, ms_y
+
+
+def time_runs(t, steps, count):
+    """Wall-clock seconds of `count` back-to-back runs of `steps` steps."""
+    els = []
+    for _ in range(count):
+        t0 = time.perf_counter()
+        t.step_n(steps)
+        t.wait()
+        els.append(time.perf_counter() - t0)
+    return els
+
+
+def repeat_stats(n, steps, els):
+    """Throughput of each back-to-back timed run: median, min and max."""
+    v = sorted(n * n * steps / e / 1e9 for e in els)
+    return {"runs": len(v), "median": round(float(np.median(v)), 3), "min": round(v[0], 3),
+            "max": round(v[-1], 3)}
 
 
 def measured_traffic(n, dtype, kernel):
@@ -511,9 +526,13 @@ def main():
                     help="two-step batch launches at this window and segment height "
                          "instead of autotuning (the same kernel in every profiling pass)")
     ap.add_argument("--tier-steps", type=int, default=2000)
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="timed runs of --steps steps; value is the first, the median, "
+                         "min and max of all are reported beside it")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="launches timed per kernel with HIP events for the roofline")
     args = ap.parse_args()
+    args.repeats = max(1, args.repeats)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU decomposition is a design, not a measured path", file=sys.stderr)
@@ -524,8 +543,9 @@ def main():
     main_dt = np.float64 if args.dtype == "f64" else np.float32
     es = np.dtype(main_dt).itemsize
     log(f"{n}x{n} {args.dtype}: {args.warmup} warm-up + {args.steps} timed steps")
-    t, el, ms_pair, ms_step, ms_x, ms_y = run_plain(le2d_amd, n, main_dt, args.steps,
-                                                    args.warmup, args.profile_steps)
+    t, els, ms_pair, ms_step, ms_x, ms_y = run_plain(le2d_amd, n, main_dt, args.steps,
+                                                     args.warmup, args.profile_steps)
+    el = els[0]
     value = n * n * args.steps / el / 1e9
     roof = roofline(n, es, ms_pair, ms_step, ms_x, ms_y, args.dtype)
     tiles = t.get_tiles() + (t.get_step_pairs(), t.get_step_pair_window())
@@ -534,8 +554,9 @@ def main():
         o_dt = np.float32 if main_dt == np.float64 else np.float64
         o_name = "f32" if o_dt == np.float32 else "f64"
         log(f"{n}x{n} {o_name}")
-        t2, el2, mp2, ms2, mx2, my2 = run_plain(le2d_amd, n, o_dt, args.steps, args.warmup,
-                                           args.profile_steps)
+        t2, els2, mp2, ms2, mx2, my2 = run_plain(le2d_amd, n, o_dt, args.steps, args.warmup,
+                                                 args.profile_steps)
+        el2 = els2[0]
         lo, hi = (t2, t) if o_dt == np.float32 else (t, t2)
         errs = rel_errors(le2d_amd.diff_norms(lo.device_view(), hi.device_view()))
         other = {"dtype": o_name, "value": round(n * n * args.steps / el2 / 1e9, 3),
@@ -543,7 +564,12 @@ def main():
                  "roofline": roofline(n, np.dtype(o_dt).itemsize, mp2, ms2, mx2, my2, o_name),
                  "fp32_vs_fp64_after_steps": args.warmup + args.steps + 4 * args.profile_steps,
                  "fp32_vs_fp64": errs}
+        # the further timed runs come after the comparison, which stays at
+        # the step count above
+        other["repeats"] = repeat_stats(n, args.steps,
+                                        els2 + time_runs(t2, args.steps, args.repeats - 1))
         t2.close()
+    els += time_runs(t, args.steps, args.repeats - 1)
     t.close()
     copy_bytes = n * n * 5 * es
     copy_gbs = 2 * copy_bytes / le2d_amd.copy_probe(copy_bytes, 10) / 1e6
@@ -579,6 +605,7 @@ def main():
                    "nx": n, "ny": n,
                    "tiles": dict(zip(("x_rows", "y_seg", "y_block", "xy_seg", "xy_block",
                                       "pair_seg", "pair_lanes"), tiles))},
+        "repeats": repeat_stats(n, args.steps, els),
         "hbm_copy_GBps": round(copy_gbs, 1),
         "roofline": roof,
         "cpu_baseline": cpu,
