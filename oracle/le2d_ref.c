@@ -22,7 +22,12 @@
  * Parity pins (tests/test_oracle.py): golden vectors written by the compiled
  * reference (tests/golden/), closed forms for the material and the ball, the
  * textbook superbee values, the zero-slope fixed point, the c=1 exact shift,
- * conservation, y-independence and brute-force 1-D advection.
+ * conservation, y-independence and brute-force 1-D advection.  The
+ * heterogeneous sweeps (orc_sweep_mat, orc_step_mat) are pinned by their
+ * reduction to orc_run with one material or identical ones, by node-wise
+ * equality with the homogeneous sweep in each node's own material, and by
+ * the normal-incidence reflection and transmission coefficients of P and S
+ * pulses at an interface.
  */
 #include <math.h>
 #include <stdio.h>
@@ -344,35 +349,48 @@ static void orc_line_mat(const orc_material *mats, const unsigned char *ids, lon
 	}
 }
 
-/* One step (x then y) of the heterogeneous scheme on an AOS or SOA grid;
- * ids[i + j*nx] < count; c1c2rho holds count (c1, c2, rho) triples. */
-int orc_step_mat(real_t *g, int nx, int ny, int soa, real_t dt, real_t hx, real_t hy, int count,
-                 const double *c1c2rho, const unsigned char *ids, int edge)
+/* One sweep along `ax` (0 = x with clamped ends, 1 = y with `edge`) of the
+ * heterogeneous scheme on an AOS or SOA grid; ids[i + j*nx] < count;
+ * c1c2rho holds count (c1, c2, rho) triples. */
+int orc_sweep_mat(real_t *g, int nx, int ny, int soa, real_t dt, real_t h, int count,
+                  const double *c1c2rho, const unsigned char *ids, int ax, int edge)
 {
 	long ns, fs;
-	int i, j, k;
+	int l, k;
 	orc_material *mats;
-	if (nx < 3 || ny < 3 || count < 1) return 1;
+	if (nx < 3 || ny < 3 || count < 1 || ax < 0 || ax > 1) return 1;
 	mats = malloc(sizeof(orc_material) * (size_t)count);
 	for (k = 0; k < count; k++)
 		orc_material_init(c1c2rho[3 * k], c1c2rho[3 * k + 1], c1c2rho[3 * k + 2], &mats[k]);
 	orc_strides(nx, ny, soa, &ns, &fs);
+	if (ax == 0) {
 #pragma omp parallel for schedule(static)
-	for (j = 0; j < ny; j++) {
-		real_t *save = malloc(sizeof(real_t) * 5 * (size_t)nx);
-		orc_line_mat(mats, ids + (long)j * nx, 1, g + (long)j * nx * ns, nx, ns, fs, 0, dt, hx,
-		             ORC_EDGE_CLAMP, save);
-		free(save);
-	}
+		for (l = 0; l < ny; l++) {
+			real_t *save = malloc(sizeof(real_t) * 5 * (size_t)nx);
+			orc_line_mat(mats, ids + (long)l * nx, 1, g + (long)l * nx * ns, nx, ns, fs, 0, dt,
+			             h, ORC_EDGE_CLAMP, save);
+			free(save);
+		}
+	} else {
 #pragma omp parallel for schedule(static)
-	for (i = 0; i < nx; i++) {
-		real_t *save = malloc(sizeof(real_t) * 5 * (size_t)ny);
-		orc_line_mat(mats, ids + i, nx, g + (long)i * ns, ny, (long)nx * ns, fs, 1, dt, hy, edge,
-		             save);
-		free(save);
+		for (l = 0; l < nx; l++) {
+			real_t *save = malloc(sizeof(real_t) * 5 * (size_t)ny);
+			orc_line_mat(mats, ids + l, nx, g + (long)l * ns, ny, (long)nx * ns, fs, 1, dt, h,
+			             edge, save);
+			free(save);
+		}
 	}
 	free(mats);
 	return 0;
+}
+
+/* One step (x then y) of the heterogeneous scheme. */
+int orc_step_mat(real_t *g, int nx, int ny, int soa, real_t dt, real_t hx, real_t hy, int count,
+                 const double *c1c2rho, const unsigned char *ids, int edge)
+{
+	int rc = orc_sweep_mat(g, nx, ny, soa, dt, hx, count, c1c2rho, ids, 0, ORC_EDGE_CLAMP);
+	if (rc) return rc;
+	return orc_sweep_mat(g, nx, ny, soa, dt, hy, count, c1c2rho, ids, 1, edge);
 }
 
 /* Size of real_t, so a loader can check which build it opened. */

@@ -54,6 +54,8 @@ class Oracle:
                                    ctypes.c_char_p]
         L.orc_step_mat.argtypes = grid_args + [R, R, R, ctypes.c_int, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_int]
+        L.orc_sweep_mat.argtypes = grid_args + [R, R, ctypes.c_int, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         assert L.orc_real_size() == self.dtype.itemsize
 
     # --- scalar pieces -----------------------------------------------------
@@ -97,29 +99,55 @@ class Oracle:
                               ctypes.byref(self._mat(mat)), steps)
         assert rc == 0
 
+    @staticmethod
+    def _table(c1c2rho, ids, nx, ny):
+        tab = np.ascontiguousarray(c1c2rho, dtype=np.float64).reshape(-1, 3)
+        ids = np.ascontiguousarray(ids, dtype=np.uint8)
+        assert ids.shape == (ny, nx) and int(ids.max()) < len(tab)
+        return tab, ids
+
     def run_mat(self, g, dt, h, c1c2rho, ids, steps, edge=EDGE_CLAMP):
         """Heterogeneous scheme: ids (ny, nx) uint8 material indices into the
         (count, 3) table of (c1, c2, rho)."""
         nx, ny, soa = self._grid(g)
-        tab = np.ascontiguousarray(c1c2rho, dtype=np.float64).reshape(-1, 3)
-        ids = np.ascontiguousarray(ids, dtype=np.uint8)
-        assert ids.shape == (ny, nx)
+        tab, ids = self._table(c1c2rho, ids, nx, ny)
         for _ in range(steps):
             rc = self.lib.orc_step_mat(_ptr(g, self.real), nx, ny, soa, dt, h[0], h[1], len(tab),
                                        tab.ctypes.data, ids.ctypes.data, edge)
             assert rc == 0
 
+    def sweep_mat(self, g, dt, h, c1c2rho, ids, axis, edge=EDGE_CLAMP):
+        """One heterogeneous sweep along x (axis 0, spacing h) or y (axis 1)."""
+        nx, ny, soa = self._grid(g)
+        tab, ids = self._table(c1c2rho, ids, nx, ny)
+        rc = self.lib.orc_sweep_mat(_ptr(g, self.real), nx, ny, soa, dt, h, len(tab),
+                                    tab.ctypes.data, ids.ctypes.data, axis, edge)
+        assert rc == 0
+
+    @staticmethod
+    def energy_density(v, rho, c1, c2):
+        """FP64 elastic energy per unit area of nodes v (..., 5) in plane strain:
+        rho/2 |v|^2 + (sxx+syy)^2 / (8 (lam+mu)) + ((sxx-syy)^2/2 + 2 sxy^2) / (4 mu),
+        mu = rho c2^2, lam = rho c1^2 - 2 mu (rho, c1, c2 scalars or arrays
+        broadcast against v[..., 0])."""
+        v = np.asarray(v, dtype=np.float64)
+        mu = rho * c2 * c2
+        lam = rho * c1 * c1 - 2.0 * mu
+        tr = v[..., 2] + v[..., 4]
+        dv = v[..., 2] - v[..., 4]
+        return (0.5 * rho * (v[..., 0] ** 2 + v[..., 1] ** 2) + tr * tr / (8.0 * (lam + mu))
+                + (0.5 * dv * dv + 2.0 * v[..., 3] ** 2) / (4.0 * mu))
+
     def energy(self, g, h, mat):
         """FP64 elastic energy of an AOS grid (homogeneous material)."""
         m = self._mat(mat)
-        v = g.reshape(-1, 5).astype(np.float64)
-        rho, c1, c2 = float(m.rho), float(m.c1), float(m.c2)
-        mu = rho * c2 * c2
-        lam = rho * c1 * c1 - 2.0 * mu
-        tr = v[:, 2] + v[:, 4]
-        dv = v[:, 2] - v[:, 4]
-        dens = (0.5 * rho * (v[:, 0] ** 2 + v[:, 1] ** 2) + tr * tr / (8.0 * (lam + mu))
-                + (0.5 * dv * dv + 2.0 * v[:, 3] ** 2) / (4.0 * mu))
+        dens = self.energy_density(g.reshape(-1, 5), float(m.rho), float(m.c1), float(m.c2))
+        return float(dens.sum()) * h[0] * h[1]
+
+    def energy_mat(self, g, h, c1c2rho, ids):
+        """FP64 elastic energy of an AOS grid with one material index per node."""
+        tab = np.asarray(c1c2rho, dtype=np.float64).reshape(-1, 3)[np.asarray(ids).reshape(-1)]
+        dens = self.energy_density(g.reshape(-1, 5), tab[:, 2], tab[:, 0], tab[:, 1])
         return float(dens.sum()) * h[0] * h[1]
 
     def set_ball(self, g, h, c, r, s):

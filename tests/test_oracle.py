@@ -269,3 +269,181 @@ def test_ring_edge_differs_from_clamp_only_on_last_two_rows():
     OD.sweep_y(b, 0.2, 1.0, (2.0, 1.0, 1.5), edge=EDGE_RING)
     assert np.array_equal(a[:-2], b[:-2])
     assert not np.array_equal(a[-2:], b[-2:])
+
+
+# --------------------------------------------------------------------------
+# 3. heterogeneous media (orc_sweep_mat / orc_step_mat) and the energy
+#    functional.  The reference has one material per task and no energy, so
+#    these are pinned by reduction to the pinned homogeneous sweeps and by
+#    closed forms of linear elasticity.
+
+MATS3 = [(2.0, 1.0, 1.0), (3.1, 1.6, 2.4), (1.4, 0.8, 1.2)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nx,ny", [(31, 20), (3, 7), (17, 40)])
+def test_mat_one_material_is_the_reference_scheme(dtype, nx, ny):
+    """One material, or three identical materials on random ids, gives the
+    bits of orc_run (golden-pinned) on AOS with the clamped edge and on SOA
+    with the ring edge."""
+    o = OD if dtype == np.float64 else OF
+    rng = np.random.default_rng(nx * ny)
+    mat = (2.3, 1.2, 1.7)
+    g0 = rng.standard_normal((ny, nx, 5)).astype(dtype)
+    ids_rand = rng.integers(0, 3, size=(ny, nx)).astype(np.uint8)
+    dt, h = 0.17, (1.0, 0.8)
+    for soa in (False, True):
+        init = np.ascontiguousarray(np.moveaxis(g0, -1, 0)) if soa else g0
+        edge = EDGE_RING if soa else EDGE_CLAMP
+        ref = init.copy()
+        o.run(ref, dt, h, mat, 4)
+        for tab, ids in (([mat], np.zeros((ny, nx), np.uint8)), ([mat] * 3, ids_rand)):
+            got = init.copy()
+            o.run_mat(got, dt, h, tab, ids, 4, edge=edge)
+            assert gd.same_bits(got, ref), (soa, len(tab))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_mat_sweep_is_the_homogeneous_sweep_in_each_nodes_material(dtype):
+    """The node-local scheme, node by node: a node's result equals the
+    homogeneous sweep of the whole grid in that node's material, at that
+    node.  Catches a stencil node read in its own (not the centre's)
+    material, an ids index transposed or shifted, or a Courant number from
+    the wrong material."""
+    o = OD if dtype == np.float64 else OF
+    rng = np.random.default_rng(41)
+    nx, ny = 29, 23
+    g0 = rng.standard_normal((ny, nx, 5)).astype(dtype)
+    ids = rng.integers(0, 3, size=(ny, nx)).astype(np.uint8)
+    ids[:, :4] = 1  # some nodes with a one-material stencil too
+    dt, hx, hy = 0.45 / 3.1, 1.0, 0.9
+    for axis, edge, h in ((0, EDGE_CLAMP, hx), (1, EDGE_CLAMP, hy), (1, EDGE_RING, hy)):
+        got = g0.copy()
+        o.sweep_mat(got, dt, h, MATS3, ids, axis, edge=edge)
+        expect = np.empty_like(g0)
+        for k, mat in enumerate(MATS3):
+            hom = g0.copy()
+            if axis == 0:
+                o.sweep_x(hom, dt, h, mat)
+            else:
+                o.sweep_y(hom, dt, h, mat, edge=edge)
+            expect[ids == k] = hom[ids == k]
+        assert gd.same_bits(got, expect), (axis, edge)
+    # a step is the x-sweep then the y-sweep
+    a, b = g0.copy(), g0.copy()
+    o.run_mat(a, dt, (hx, hy), MATS3, ids, 1)
+    o.sweep_mat(b, dt, hx, MATS3, ids, 0)
+    o.sweep_mat(b, dt, hy, MATS3, ids, 1)
+    assert gd.same_bits(a, b)
+
+
+def _plane_pulse(m1, m2, wave, nx=1600, xi=800, sigma=12.0):
+    """A right-going P (vx, sxx = -rho c1 vx, syy = lam/(lam+2mu) sxx) or S
+    (vy, sxy = -rho c2 vy) Gaussian pulse in material m1, 250 cells left of
+    an interface with m2 at column xi; three identical rows (y-sweeps are
+    exact no-ops).  Runs until the pulse has left the interface and returns
+    (integral of the incident velocity, reflected integral, transmitted
+    integral, speed in m1, speed in m2, Z1, Z2)."""
+    ny = 3
+    ids = np.zeros((ny, nx), np.uint8)
+    ids[:, xi:] = 1
+    dt = 0.4 / max(m1[0], m2[0])
+    x = np.arange(nx)
+    f = np.exp(-0.5 * ((x - (xi - 250)) / sigma) ** 2)
+    c1, c2, rho = m1
+    mu = rho * c2 * c2
+    lam = rho * c1 * c1 - 2 * mu
+    g = np.zeros((ny, nx, 5))
+    if wave == "P":
+        q, ca, cb = 0, c1, m2[0]
+        g[:, :, 0] = f
+        g[:, :, 2] = -rho * c1 * f
+        g[:, :, 4] = lam / (lam + 2 * mu) * g[:, :, 2]
+    else:
+        q, ca, cb = 1, c2, m2[1]
+        g[:, :, 1] = f
+        g[:, :, 3] = -rho * c2 * f
+    i0 = g[1, :, q].sum()
+    steps = int(np.ceil(480 / (ca * dt)))  # centre 230 cells past the interface
+    OD.run_mat(g, dt, (1.0, 1.0), [m1, m2], ids, steps)
+    assert np.array_equal(g[0], g[1]) and np.array_equal(g[1], g[2])
+    v = g[1, :, q]
+    # pulses clear of the interface (a residue of the limiter's eps, ~1e-6, stays)
+    assert abs(v[xi - 40:xi + 40]).max() < 1e-5 * abs(v).max()
+    assert abs(v[:3]).max() < 1e-12 and abs(v[-3:]).max() < 1e-12  # and of the edges
+    return i0, v[:xi].sum(), v[xi:].sum(), ca, cb, m1[2] * ca, m2[2] * cb
+
+
+@pytest.mark.parametrize("wave", ["P", "S"])
+@pytest.mark.parametrize("m1,m2", [((2.0, 1.0, 1.0), (3.0, 1.5, 2.5)),
+                                   ((3.0, 1.5, 2.5), (2.0, 1.0, 1.0)),
+                                   ((2.4376, 1.2444, 2.5514), (1.8378, 1.0685, 2.7471))])
+def test_mat_normal_incidence_reflection_and_transmission(wave, m1, m2):
+    """Velocity and traction continuity at a welded interface give, for a
+    normally incident plane wave, reflected / incident velocity
+    R = (Z1 - Z2) / (Z1 + Z2) and transmitted / incident T = 2 Z1 / (Z1 + Z2)
+    with Z = rho c (c1 for P, c2 for S).  The scheme is conservative away
+    from the interface, so the velocity integrals of the separated pulses
+    carry these ratios (the transmitted pulse is stretched by c_2 / c_1).
+    A wrong impedance, a swapped wave speed or a sign slip in the node-local
+    projection moves the ratios by far more than the 2e-3 allowed for the
+    interface's discretisation error (measured: below 6e-4)."""
+    i0, ir, it, ca, cb, z1, z2 = _plane_pulse(m1, m2, wave)
+    r_exact = (z1 - z2) / (z1 + z2)
+    t_exact = 2 * z1 / (z1 + z2)
+    assert ir / i0 == pytest.approx(r_exact, abs=2e-3)
+    assert it / i0 == pytest.approx(t_exact * cb / ca, abs=2e-3)
+
+
+def _lame(c1, c2, rho):
+    mu = rho * c2 * c2
+    return rho * c1 * c1 - 2 * mu, mu
+
+
+@pytest.mark.parametrize("mat", [(2.0, 1.0, 1.0), (3.1, 1.6, 2.4), (2.4376, 1.2444, 2.5514)])
+def test_energy_closed_forms(mat):
+    """Plane-strain energy of states whose value linear elasticity fixes.
+    (1) Plane P wave (vx = f, sxx = -rho c1 f, syy = lam/(lam+2mu) sxx):
+    strain energy equals kinetic energy.  (2) Plane S wave (vy = f,
+    sxy = -rho c2 f): the same.  (3) Uniform pressure sxx = syy = s:
+    s^2 / (2 (lam + mu)) per unit area.  (4) Pure shear sxy = t: t^2 / (2 mu).
+    A lam+mu / lam+2mu slip or a dropped factor breaks one of them."""
+    c1, c2, rho = mat
+    lam, mu = _lame(c1, c2, rho)
+    rng = np.random.default_rng(12)
+    f = rng.standard_normal((7, 9))
+    h = (0.7, 1.3)
+    area = h[0] * h[1]
+    kinetic = 0.5 * rho * float((f * f).sum()) * area
+    g = np.zeros((7, 9, 5))
+    g[..., 0] = f
+    g[..., 2] = -rho * c1 * f
+    g[..., 4] = lam / (lam + 2 * mu) * g[..., 2]
+    assert OD.energy(g, h, mat) == pytest.approx(2 * kinetic, rel=1e-13)
+    g = np.zeros((7, 9, 5))
+    g[..., 1] = f
+    g[..., 3] = -rho * c2 * f
+    assert OD.energy(g, h, mat) == pytest.approx(2 * kinetic, rel=1e-13)
+    g = np.zeros((7, 9, 5))
+    g[..., 2] = g[..., 4] = 0.37
+    assert OD.energy(g, h, mat) == pytest.approx(63 * 0.37 ** 2 / (2 * (lam + mu)) * area,
+                                                 rel=1e-13)
+    g = np.zeros((7, 9, 5))
+    g[..., 3] = -1.9
+    assert OD.energy(g, h, mat) == pytest.approx(63 * 1.9 ** 2 / (2 * mu) * area, rel=1e-13)
+
+
+def test_energy_with_a_material_plane():
+    """Per-node materials: identical materials give the homogeneous energy;
+    distinct ones give the sum of each material's energy over its nodes."""
+    rng = np.random.default_rng(13)
+    g = rng.standard_normal((11, 14, 5))
+    ids = rng.integers(0, 3, size=(11, 14)).astype(np.uint8)
+    h = (1.0, 0.5)
+    assert OD.energy_mat(g, h, [MATS3[1]] * 3, ids) == pytest.approx(
+        OD.energy(g, h, MATS3[1]), rel=1e-14)
+    parts = 0.0
+    for k, mat in enumerate(MATS3):
+        gk = np.where((ids == k)[..., None], g, 0.0)
+        parts += OD.energy(np.ascontiguousarray(gk), h, mat)
+    assert OD.energy_mat(g, h, MATS3, ids) == pytest.approx(parts, rel=1e-14)
