@@ -34,6 +34,7 @@ extern "C" {
 #define LE2D_ERR_NOMEM 3
 #define LE2D_ERR_STATE 4
 #define LE2D_ERR_BUDGET 5
+#define LE2D_ERR_BUSY 6
 
 /* When le_step returns.  WAIT (default): after the step has finished on the
  * GPU.  ASYNC: after the launches are queued.  COHERENT: after the step has
@@ -44,6 +45,19 @@ extern "C" {
 #define LE2D_SYNC_COHERENT 2
 
 const char *le2d_last_error(void);
+
+/* Device memory.  Every device allocation the library makes (state buffers,
+ * batch and server arenas, tile and material tables, reduction partials)
+ * goes through `alloc`, and is returned through `release` of the allocator
+ * that made it, whatever is installed by then.  alloc returns 0 and a device
+ * pointer in *ptr, or nonzero to refuse (the library call then fails with
+ * LE2D_ERR_NOMEM and leaves its object as it was).  Both NULL: hipMalloc /
+ * hipFree (the default).  Each of liblecore_d.so and liblecore_f.so keeps
+ * its own setting.  No counterpart in the reference, which mallocs host
+ * memory and does not check the result (simple/le_core.c:118-121). */
+typedef int (*le2d_alloc_fn)(void **ptr, size_t bytes, void *ctx);
+typedef void (*le2d_free_fn)(void *ptr, void *ctx);
+int le2d_set_device_allocator(le2d_alloc_fn alloc, le2d_free_fn release, void *ctx);
 
 int le2d_set_sync_mode(le_task *task, int mode);
 /* Block until all queued work of the task has finished. */
@@ -224,6 +238,8 @@ typedef struct {
   double node_updates;   /* node-updates executed so far */
   double busy_seconds;   /* wall time the engine spent in chunks */
   size_t bytes_in_use, peak_bytes;
+  long long cancelled;   /* requests cancelled (queued, live or kept results) */
+  long long rejected;    /* submits refused with LE2D_ERR_BUSY */
 } le2d_server_stats;
 
 typedef struct le2d_server le2d_server;
@@ -242,15 +258,32 @@ int le2d_server_set_pairs(le2d_server *srv, int pair_seg);
 /* Lanes per two-step workgroup, 64, 128 or 256; 0 (the default): 256 for
  * FP32 requests and 64 for FP64.  Results do not change. */
 int le2d_server_set_pair_window(le2d_server *srv, int lanes);
+/* Admission.  max_queue: requests that may wait (0, the default: no limit);
+ * at the limit le2d_server_submit returns -LE2D_ERR_BUSY and the caller
+ * retries later (back-pressure).  Admission takes queued requests in
+ * arrival order while slots are free; among the first `window` (default 64)
+ * a request that does not fit the budget now is passed by later ones that
+ * do, until it has been passed `max_pass` times (default 16; 0: strict
+ * arrival order), after which admission waits for it. */
+int le2d_server_set_admission(le2d_server *srv, int max_queue, int window, int max_pass);
 /* Queue a request; returns its id (>= 0) or a negative error code
- * (-LE2D_ERR_BUDGET when it can never fit). */
+ * (-LE2D_ERR_BUDGET when it can never fit, -LE2D_ERR_BUSY when the queue is
+ * at its limit). */
 long long le2d_server_submit(le2d_server *srv, const le2d_request *req);
-/* Block until request `id` has finished (timeout_s < 0: no limit). */
+/* Cancel a request: a queued one leaves the queue at once; a live one stops
+ * at the end of the current chunk of steps and its memory returns to the
+ * budget; a finished one's kept result is discarded.  le2d_server_wait on a
+ * cancelled request returns LE2D_ERR_STATE.  LE2D_ERR_STATE also when the
+ * request has already been released or cancelled. */
+int le2d_server_cancel(le2d_server *srv, long long id);
+/* Block until request `id` has finished or was cancelled (timeout_s < 0: no
+ * limit). */
 int le2d_server_wait(le2d_server *srv, long long id, double timeout_s);
 /* Block until every submitted request has finished. */
 int le2d_server_drain(le2d_server *srv, double timeout_s);
 /* Copy a kept result into an AOS host array (nx*ny nodes of five reals of
- * the request's type) and release its device memory. */
+ * the request's type) and release its device memory.  Only one call per
+ * result succeeds; a concurrent second call returns LE2D_ERR_STATE. */
 int le2d_server_result(le2d_server *srv, long long id, void *aos);
 /* Submission, admission and completion times (seconds, steady clock). */
 int le2d_server_times(le2d_server *srv, long long id, double *t_submit, double *t_admit,

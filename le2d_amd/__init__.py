@@ -19,7 +19,7 @@ LIBDIR = os.path.join(HERE, "lib")
 
 SYNC_WAIT, SYNC_ASYNC, SYNC_COHERENT = 0, 1, 2
 # LE2D_ERR_* status codes of include/le2d.h
-ERR_ARG, ERR_DEVICE, ERR_NOMEM, ERR_STATE, ERR_BUDGET = 1, 2, 3, 4, 5
+ERR_ARG, ERR_DEVICE, ERR_NOMEM, ERR_STATE, ERR_BUDGET, ERR_BUSY = 1, 2, 3, 4, 5, 6
 
 # Symbols declared in include/le2d.h (extensions), beyond simple/le_core.h.
 _EXT = {
@@ -67,7 +67,15 @@ class le2d_server_stats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_longlong), ("queued", ctypes.c_longlong),
                 ("live", ctypes.c_int), ("peak_live", ctypes.c_int),
                 ("node_updates", ctypes.c_double), ("busy_seconds", ctypes.c_double),
-                ("bytes_in_use", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t)]
+                ("bytes_in_use", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t),
+                ("cancelled", ctypes.c_longlong), ("rejected", ctypes.c_longlong)]
+
+
+# include/le2d.h device allocator hook: int alloc(void **ptr, size_t bytes,
+# void *ctx) and void release(void *ptr, void *ctx)
+ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t,
+                            ctypes.c_void_p)
+FREE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p)
 
 
 # include/le2d.h batched-instance API (no le_task argument).
@@ -100,7 +108,11 @@ _BATCH = {
     "le2d_server_set_shape": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "le2d_server_set_pairs": ([_VP, ctypes.c_int], ctypes.c_int),
     "le2d_server_set_pair_window": ([_VP, ctypes.c_int], ctypes.c_int),
+    "le2d_server_set_admission": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+                                  ctypes.c_int),
     "le2d_server_submit": ([_VP, ctypes.POINTER(le2d_request)], ctypes.c_longlong),
+    "le2d_server_cancel": ([_VP, ctypes.c_longlong], ctypes.c_int),
+    "le2d_set_device_allocator": ([_VP, _VP, _VP], ctypes.c_int),
     "le2d_server_wait": ([_VP, ctypes.c_longlong, ctypes.c_double], ctypes.c_int),
     "le2d_server_drain": ([_VP, ctypes.c_double], ctypes.c_int),
     "le2d_server_result": ([_VP, ctypes.c_longlong, _VP], ctypes.c_int),
@@ -145,6 +157,23 @@ def load(dtype=np.float64):
 
 class LE2DError(RuntimeError):
     pass
+
+
+class BusyError(LE2DError):
+    """The server's queue is at its limit (LE2D_ERR_BUSY): retry later."""
+
+
+def set_device_allocator(alloc=None, release=None, dtype=np.float64):
+    """Install device allocation callbacks (ALLOC_FN, FREE_FN instances) in
+    the library of `dtype`, or restore hipMalloc / hipFree with None.  The
+    caller keeps the callback objects alive while any allocation they made
+    is in use."""
+    lib = load(dtype)
+    a = ctypes.cast(alloc, ctypes.c_void_p) if alloc is not None else None
+    r = ctypes.cast(release, ctypes.c_void_p) if release is not None else None
+    rc = lib.le2d_set_device_allocator(a, r, None)
+    if rc != 0:
+        raise LE2DError(f"set_device_allocator: {rc} {lib.le2d_last_error().decode()}")
 
 
 def copy_probe(nbytes, iters=20, dtype=np.float64):
@@ -486,6 +515,16 @@ class Server:
         requests, 64 for FP64)."""
         self._check(self.lib.le2d_server_set_pair_window(self.h, lanes), "set_pair_window")
 
+    def set_admission(self, max_queue=0, window=64, max_pass=16):
+        """Queue limit (0: none; submit then raises BusyError at the limit),
+        the queued requests admission scans, and how often a request that
+        does not fit may be passed by later ones."""
+        self._check(self.lib.le2d_server_set_admission(self.h, max_queue, window, max_pass),
+                    "set_admission")
+
+    def cancel(self, i):
+        self._check(self.lib.le2d_server_cancel(self.h, i), "le2d_server_cancel")
+
     def submit(self, nx, ny, steps, dtype=np.float32, dt=0.25, h=(1.0, 1.0),
                mat=(2.0, 1.0, 1.0), disk=(None, None, 10.0, 1.0), keep=False):
         cx, cy, r, sv = disk
@@ -493,6 +532,8 @@ class Server:
                          mat[1], mat[2], nx / 2 if cx is None else cx,
                          ny / 2 if cy is None else cy, r, sv, int(keep))
         i = self.lib.le2d_server_submit(self.h, ctypes.byref(q))
+        if i == -ERR_BUSY:
+            raise BusyError("le2d_server_submit: " + self.lib.le2d_last_error().decode())
         if i < 0:
             self._check(-i, "le2d_server_submit")
         self.shapes[i] = ((ny, nx), np.dtype(dtype))

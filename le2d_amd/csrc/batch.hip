@@ -96,10 +96,10 @@ struct BatchImpl {
   ~BatchImpl() {
     drop_graphs();
     if (stream) (void)hipStreamSynchronize(stream);
-    if (d_tiles) (void)hipFree(d_tiles);
-    if (d_tiles1) (void)hipFree(d_tiles1);
-    if (d_inst) (void)hipFree(d_inst);
-    if (arena) (void)hipFree(arena);
+    dev_free(d_tiles);
+    dev_free(d_tiles1);
+    dev_free(d_inst);
+    dev_free(arena);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -139,8 +139,10 @@ struct BatchImpl {
       inst_coef<T>(in, &h[i].cx, &h[i].cy);
       h[i].par = 0;
     }
-    if (!d_inst && hipMalloc(&d_inst, h.size() * sizeof(InstDesc<T>)) != hipSuccess)
-      return set_error(ERR_NOMEM, "batch: descriptor table");
+    if (!d_inst) {
+      const int rc = dev_alloc(&d_inst, h.size() * sizeof(InstDesc<T>), "batch: descriptor table");
+      if (rc) return rc;
+    }
     return hcheck(hipMemcpy(d_inst, h.data(), h.size() * sizeof(InstDesc<T>), hipMemcpyHostToDevice),
                   "batch: descriptors");
   }
@@ -158,10 +160,11 @@ struct BatchImpl {
         for (int st = 0; st < strips; ++st)
           h.push_back({(int)i, st, j0, std::min(j0 + seg, in.ny), j0 / seg});
     }
-    if (*d) (void)hipFree(*d);
+    dev_free(*d);
     *d = nullptr;
-    if (hipMalloc(d, std::max<size_t>(1, h.size()) * sizeof(TileJob)) != hipSuccess)
-      return set_error(ERR_NOMEM, "batch: tile table");
+    const int rc = dev_alloc((void **)d, std::max<size_t>(1, h.size()) * sizeof(TileJob),
+                             "batch: tile table");
+    if (rc) return rc;
     *n = (int)h.size();
     return hcheck(hipMemcpy(*d, h.data(), h.size() * sizeof(TileJob), hipMemcpyHostToDevice),
                   "batch: tiles");
@@ -342,10 +345,16 @@ int le2d_batch_commit(le2d_batch *b) {
       off += (bytes + 4095) / 4096 * 4096;
     }
   }
-  if (hipMalloc(&B.arena, off) != hipSuccess)
-    return le2d::set_error(le2d::ERR_NOMEM, "batch: arena of %zu bytes", off);
+  (void)hipStreamSynchronize(B.stream);
+  le2d::dev_free(B.arena);  // a commit that failed part way left these
+  le2d::dev_free(B.d_inst);
+  B.d_inst = nullptr;
+  B.arena = nullptr;
+  B.arena_bytes = 0;
+  int rc = le2d::dev_alloc((void **)&B.arena, off, "batch: arena");
+  if (rc) return rc;
   B.arena_bytes = off;
-  int rc = le2d::hcheck(hipMemsetAsync(B.arena, 0, off, B.stream), "batch: clear");
+  rc = le2d::hcheck(hipMemsetAsync(B.arena, 0, off, B.stream), "batch: clear");
   if (rc) return rc;
   rc = B.elem == 8 ? B.upload_desc<double>() : B.upload_desc<float>();
   if (rc) return rc;

@@ -53,12 +53,29 @@ struct DevGrid {
 
 // Error reporting: every function returns 0 on success or a nonzero code and
 // records a message retrievable with last_error().
-enum Status { OK = 0, ERR_ARG = 1, ERR_DEVICE = 2, ERR_NOMEM = 3, ERR_STATE = 4, ERR_BUDGET = 5 };
+enum Status {
+  OK = 0,
+  ERR_ARG = 1,
+  ERR_DEVICE = 2,
+  ERR_NOMEM = 3,
+  ERR_STATE = 4,
+  ERR_BUDGET = 5,
+  ERR_BUSY = 6
+};
 int set_error(int code, const char *fmt, ...);
 const char *last_error();
 
 // Aborts with a message when no usable HIP device is present.
 int require_device();
+
+// Every device allocation of the library goes through dev_alloc / dev_free:
+// the caller's allocator when one is installed (le2d_set_device_allocator),
+// else hipMalloc / hipFree.  An allocation is freed by the allocator that
+// made it, whatever is installed by then.  dev_alloc returns OK, or
+// ERR_NOMEM with *p = nullptr and the error message set (`what` names the
+// allocation).
+int dev_alloc(void **p, size_t bytes, const char *what);
+void dev_free(void *p);
 
 // Layout of one state buffer: field q of node (i, j) at q*plane + j*pitch + i.
 void grid_layout(int elem, int nx, int ny, long *pitch, long *plane);

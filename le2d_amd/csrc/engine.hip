@@ -6,7 +6,11 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <mutex>
+#include <unordered_map>
+
 #include "engine.h"
+#include "le2d.h"
 
 namespace le2d {
 
@@ -33,6 +37,77 @@ int require_device() {
     abort();
   }
   return OK;
+}
+
+// Device allocator.  Allocations made by an installed allocator are recorded
+// with its free function, so each is returned to the allocator that made it.
+namespace {
+struct Allocator {
+  le2d_alloc_fn alloc = nullptr;
+  le2d_free_fn release = nullptr;
+  void *ctx = nullptr;
+};
+struct Owner {
+  le2d_free_fn release;
+  void *ctx;
+};
+std::mutex g_alloc_mu;
+Allocator g_alloc;
+std::unordered_map<void *, Owner> g_owned;
+}  // namespace
+
+int dev_alloc(void **p, size_t bytes, const char *what) {
+  *p = nullptr;
+  Allocator a;
+  {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    a = g_alloc;
+  }
+  if (a.alloc) {
+    void *q = nullptr;
+    if (a.alloc(&q, bytes, a.ctx) != 0 || !q)
+      return set_error(ERR_NOMEM, "%s: the installed allocator refused %zu bytes", what, bytes);
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    g_owned[q] = {a.release, a.ctx};
+    *p = q;
+    return OK;
+  }
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(ERR_NOMEM, "%s: hipMalloc(%zu) failed: %s", what, bytes,
+                     hipGetErrorString(e));
+  }
+  return OK;
+}
+
+}  // namespace le2d
+
+extern "C" int le2d_set_device_allocator(le2d_alloc_fn alloc, le2d_free_fn release, void *ctx) {
+  if ((alloc == nullptr) != (release == nullptr))
+    return le2d::set_error(le2d::ERR_ARG, "set_device_allocator: give both functions or neither");
+  std::lock_guard<std::mutex> lk(le2d::g_alloc_mu);
+  le2d::g_alloc = {alloc, release, alloc ? ctx : nullptr};
+  return le2d::OK;
+}
+
+namespace le2d {
+
+void dev_free(void *p) {
+  if (!p) return;
+  Owner o{nullptr, nullptr};
+  {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    auto it = g_owned.find(p);
+    if (it != g_owned.end()) {
+      o = it->second;
+      g_owned.erase(it);
+    }
+  }
+  if (o.release)
+    o.release(p, o.ctx);
+  else
+    (void)hipFree(p);
 }
 
 // One grid row of one field, padded to 256 bytes so every row starts aligned.
@@ -75,12 +150,10 @@ int grid_alloc(DevGrid *g, int elem, int nx, int ny, hipStream_t stream) {
   (void)hipGetDevice(&g->device);
   const size_t bytes = grid_bytes(elem, nx, ny);
   for (int k = 0; k < 2; ++k) {
-    hipError_t e = hipMalloc(&g->buf[k], bytes);
-    if (e != hipSuccess) {
-      g->buf[k] = nullptr;
+    const int rc = dev_alloc(&g->buf[k], bytes, "grid_alloc");
+    if (rc) {
       grid_free(g);
-      return set_error(ERR_NOMEM, "grid_alloc: hipMalloc(%zu) failed: %s", bytes,
-                       hipGetErrorString(e));
+      return rc;
     }
   }
   g->tiles = default_tiles(elem, nx, ny);
@@ -89,7 +162,7 @@ int grid_alloc(DevGrid *g, int elem, int nx, int ny, hipStream_t stream) {
 
 void grid_free(DevGrid *g) {
   for (int k = 0; k < 2; ++k) {
-    if (g->buf[k]) (void)hipFree(g->buf[k]);
+    dev_free(g->buf[k]);
     g->buf[k] = nullptr;
   }
 }
@@ -299,9 +372,11 @@ int copy_probe(size_t bytes, int iters, double *ms) {
   require_device();
   if (bytes < 16 || iters < 1) return set_error(ERR_ARG, "copy_probe: bytes %zu iters %d", bytes, iters);
   void *a = nullptr, *b = nullptr;
-  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
-    if (a) (void)hipFree(a);
-    return set_error(ERR_NOMEM, "copy_probe: hipMalloc(%zu)", bytes);
+  int rc = dev_alloc(&a, bytes, "copy_probe");
+  if (!rc) rc = dev_alloc(&b, bytes, "copy_probe");
+  if (rc) {
+    dev_free(a);
+    return rc;
   }
   (void)hipMemset(a, 1, bytes);
   const long n = (long)(bytes / 16);
@@ -319,8 +394,8 @@ int copy_probe(size_t bytes, int iters, double *ms) {
   (void)hipEventElapsedTime(&t, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipFree(a);
-  (void)hipFree(b);
+  dev_free(a);
+  dev_free(b);
   if (e != hipSuccess) return set_error(ERR_DEVICE, "copy_probe: %s", hipGetErrorString(e));
   *ms = t / iters;
   return OK;

@@ -153,8 +153,8 @@ int diff_norms(const void *a, int ea, long pa, long qa, const void *b, int eb, l
                int nx, int ny, hipStream_t stream, double out[4]) {
   const int blocks = 1024;
   double *partial = nullptr;
-  if (hipMalloc((void **)&partial, 4 * blocks * sizeof(double)) != hipSuccess)
-    return set_error(ERR_NOMEM, "diff_norms");
+  const int arc = dev_alloc((void **)&partial, 4 * blocks * sizeof(double), "diff_norms");
+  if (arc) return arc;
   dim3 g(blocks), bl(256);
   if (ea == 8 && eb == 8)
     hipLaunchKernelGGL((k_diff_norms<double, double>), g, bl, 0, stream, (const double *)a, pa, qa,
@@ -173,7 +173,7 @@ int diff_norms(const void *a, int ea, long pa, long qa, const void *b, int eb, l
   if (e == hipSuccess)
     e = hipMemcpyAsync(h.data(), partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  (void)hipFree(partial);
+  dev_free(partial);
   if (e != hipSuccess) return set_error(ERR_DEVICE, "diff_norms: %s", hipGetErrorString(e));
   double s2 = 0, r2 = 0, sm = 0, rm = 0;
   for (int k = 0; k < blocks; ++k) {
@@ -207,7 +207,7 @@ void mat_release(MatPlane *m) {
   m->ev_fork = m->ev_join = nullptr;
   for (void *p : {(void *)m->d_ids, m->d_cx, m->d_cy, (void *)m->d_homog, (void *)m->d_mixed,
                   (void *)m->d_homog2, (void *)m->d_mixed2, (void *)m->d_etab})
-    if (p) (void)hipFree(p);
+    dev_free(p);
   m->d_ids = nullptr;
   m->d_cx = m->d_cy = nullptr;
   m->d_homog = m->d_mixed = m->d_homog2 = m->d_mixed2 = nullptr;
@@ -247,11 +247,12 @@ static int upload_coefs(MatPlane *m, double dt, double hx, double hy) {
     et[3 * k + 1] = rho * c1 * c1 - 2.0 * et[3 * k + 2];
   }
   const size_t b = m->count * sizeof(SweepCoef<T>);
-  if (!m->d_cx && hipMalloc(&m->d_cx, b) != hipSuccess) return set_error(ERR_NOMEM, "materials");
-  if (!m->d_cy && hipMalloc(&m->d_cy, b) != hipSuccess) return set_error(ERR_NOMEM, "materials");
-  if (!m->d_etab && hipMalloc(&m->d_etab, et.size() * sizeof(double)) != hipSuccess)
-    return set_error(ERR_NOMEM, "materials");
-  int rc = mcheck(hipMemcpy(m->d_cx, cx.data(), b, hipMemcpyHostToDevice), "materials");
+  int rc = OK;
+  if (!m->d_cx) rc = dev_alloc(&m->d_cx, b, "materials");
+  if (!rc && !m->d_cy) rc = dev_alloc(&m->d_cy, b, "materials");
+  if (!rc && !m->d_etab) rc = dev_alloc((void **)&m->d_etab, et.size() * sizeof(double), "materials");
+  if (rc) return rc;
+  rc = mcheck(hipMemcpy(m->d_cx, cx.data(), b, hipMemcpyHostToDevice), "materials");
   if (!rc) rc = mcheck(hipMemcpy(m->d_cy, cy.data(), b, hipMemcpyHostToDevice), "materials");
   if (!rc)
     rc = mcheck(hipMemcpy(m->d_etab, et.data(), et.size() * sizeof(double), hipMemcpyHostToDevice),
@@ -274,9 +275,9 @@ int mat_set(DevGrid *g, MatPlane *m, int count, const double *c1c2rho, const uns
   m->ids.assign(n, 0);
   if (ids) m->ids.assign(ids, ids + n);
   m->idp = (g->nx + 255) / 256 * 256;
-  if (hipMalloc((void **)&m->d_ids, (size_t)m->idp * g->ny) != hipSuccess)
-    return set_error(ERR_NOMEM, "materials: id plane");
-  int rc = mcheck(hipMemcpy2D(m->d_ids, m->idp, m->ids.data(), g->nx, g->nx, g->ny,
+  int rc = dev_alloc((void **)&m->d_ids, (size_t)m->idp * g->ny, "materials: id plane");
+  if (rc) return rc;
+  rc = mcheck(hipMemcpy2D(m->d_ids, m->idp, m->ids.data(), g->nx, g->nx, g->ny,
                               hipMemcpyHostToDevice),
                   "materials: ids");
   if (rc) return rc;
@@ -319,13 +320,13 @@ static void split_tiles(const MatPlane *m, int nx, int ny, int width, int ch, in
 }
 
 static int upload_tiles(const std::vector<MatTile> &v, MatTile **d, int *n) {
-  if (*d) (void)hipFree(*d);
+  dev_free(*d);
   *d = nullptr;
   *n = 0;
   if (v.empty()) return OK;
-  if (hipMalloc((void **)d, v.size() * sizeof(MatTile)) != hipSuccess)
-    return set_error(ERR_NOMEM, "materials: tiles");
-  int rc = mcheck(hipMemcpy(*d, v.data(), v.size() * sizeof(MatTile), hipMemcpyHostToDevice),
+  int rc = dev_alloc((void **)d, v.size() * sizeof(MatTile), "materials: tiles");
+  if (rc) return rc;
+  rc = mcheck(hipMemcpy(*d, v.data(), v.size() * sizeof(MatTile), hipMemcpyHostToDevice),
                   "materials: tiles");
   if (rc) return rc;
   *n = (int)v.size();
@@ -485,16 +486,20 @@ int energy(DevGrid *g, const MatPlane *m, double rho, double c1, double c2, doub
   if (!tab) {
     const double mu = rho * c2 * c2;
     const double host[3] = {rho, rho * c1 * c1 - 2.0 * mu, mu};
-    if (hipMalloc((void **)&etab, sizeof host) != hipSuccess) return set_error(ERR_NOMEM, "energy");
+    const int arc = dev_alloc((void **)&etab, sizeof host, "energy: table");
+    if (arc) return arc;
     if (hipMemcpy(etab, host, sizeof host, hipMemcpyHostToDevice) != hipSuccess) {
-      (void)hipFree(etab);
+      dev_free(etab);
       return set_error(ERR_DEVICE, "energy: table");
     }
     tab = etab;
   }
-  if (hipMalloc((void **)&partial, blocks * sizeof(double)) != hipSuccess) {
-    if (etab) (void)hipFree(etab);
-    return set_error(ERR_NOMEM, "energy");
+  {
+    const int arc = dev_alloc((void **)&partial, blocks * sizeof(double), "energy");
+    if (arc) {
+      dev_free(etab);
+      return arc;
+    }
   }
   const unsigned char *ids = m && m->active ? m->d_ids : nullptr;
   const long idp = m && m->active ? m->idp : 0;
@@ -511,8 +516,8 @@ int energy(DevGrid *g, const MatPlane *m, double rho, double c1, double c2, doub
   if (!rc) rc = mcheck(hipMemcpyAsync(h.data(), partial, blocks * sizeof(double),
                                       hipMemcpyDeviceToHost, g->stream), "energy");
   if (!rc) rc = mcheck(hipStreamSynchronize(g->stream), "energy");
-  (void)hipFree(partial);
-  if (etab) (void)hipFree(etab);
+  dev_free(partial);
+  dev_free(etab);
   if (rc) return rc;
   double s = 0.0;
   for (double v : h) s += v;

@@ -5,7 +5,9 @@
 // A request is one grid (size, material, disk source, step count, FP32 or
 // FP64).  An engine thread runs chunks: it admits queued requests in arrival
 // order while the budget (a device arena sub-allocated first-fit) and the
-// slot count allow, initialises their fields on the device, then launches
+// slot count allow -- a request that does not fit yet may be passed by later
+// ones that do, a bounded number of times -- initialises their fields on
+// the device, then launches
 // `chunk` steps in which every live request advances by one step per launch
 // -- one k_step_batch_pairs launch (one workgroup per tile, two steps) per
 // precision per two steps, and a k_step_batch_tiles launch for an odd last
@@ -85,11 +87,17 @@ class Arena {
   size_t used_ = 0;
 };
 
-enum ReqState { QUEUED = 0, LIVE = 1, DONE = 2, RELEASED = 3 };
+// States in order: a request is finished from DONE on.  RELEASING: a
+// result download holds the request (a second le2d_server_result or a
+// cancel finds it busy).  CANCELLED: removed before or while it ran, or its
+// kept result discarded.
+enum ReqState { QUEUED = 0, LIVE = 1, DONE = 2, RELEASING = 3, RELEASED = 4, CANCELLED = 5 };
 
 struct Req {
   le2d_request spec;
   int state = QUEUED;
+  int passed = 0;       // later requests admitted ahead of this one
+  bool cancel = false;  // cancelled while live: dropped at the end of the chunk
   double t_submit = 0, t_admit = 0, t_done = 0;
   size_t off = 0, bytes = 0;
   long pitch = 0, plane = 0;
@@ -104,6 +112,11 @@ struct Server {
   int max_live;
   int plain;
   int max_chunk = 16;
+  // queue limit (0: none; submit returns -LE2D_ERR_BUSY at the limit), the
+  // queued requests admission looks at, and how often one that does not
+  // fit may be passed by later ones before admission waits for it
+  int max_queue = 0;
+  int window = 64, max_pass = 16;
   int fb = 256, seg = 16;
   int pair_seg = 64;  // two-step launches: largest segment height; 0: one step per launch
   // lanes per two-step workgroup (64, 128 or 256); 0: 256 for FP32 requests,
@@ -141,7 +154,7 @@ struct Server {
   std::vector<int> pre1[2], pre2[2];
   bool dirty = true;
   // statistics
-  long long completed = 0;
+  long long completed = 0, cancelled = 0, rejected = 0;
   double node_updates = 0, busy_s = 0;
   size_t peak_bytes = 0;
   int peak_live = 0;
@@ -160,19 +173,31 @@ struct Server {
     cv_done.notify_all();
   }
 
-  // Admit queued requests in order while they fit (caller holds mu).
+  // Admit queued requests in arrival order while slots are free (caller
+  // holds mu).  Among the first `window` queued requests, one that does not
+  // fit the arena now is passed by later ones that do, until it has been
+  // passed `max_pass` times; from then on admission waits for it, so no
+  // request waits for ever behind a stream of smaller ones.
   void admit_locked() {
-    while (!queue.empty() && (int)live.size() < max_live) {
-      Req &r = reqs[queue.front()];
+    int seen = 0;
+    for (auto it = queue.begin(); it != queue.end() && (int)live.size() < max_live &&
+                                  seen < window;
+         ++seen) {
+      Req &r = reqs[*it];
       size_t off = 0;
-      if (!arena.alloc(r.bytes, &off)) break;
+      if (!arena.alloc(r.bytes, &off)) {
+        if (r.passed >= max_pass) break;
+        ++it;
+        continue;
+      }
+      for (auto jt = queue.begin(); jt != it; ++jt) ++reqs[*jt].passed;
       r.off = off;
       r.state = LIVE;
       r.t_admit = now_s();
       r.cur = 0;
       r.left = r.spec.steps;
-      live.push_back(queue.front());
-      queue.pop_front();
+      live.push_back(*it);
+      it = queue.erase(it);
       dirty = true;
       init_fields(r);
     }
@@ -199,12 +224,17 @@ struct Server {
 
   char *buf(const Req &r, int k) { return arena_dev + r.off + k * (r.bytes / 2); }
 
-  // Device copy of a tile table, grown as needed (in stream order).
+  // Device copy of a tile table, grown as needed (in stream order: the
+  // stream is idle whenever the tables are rebuilt).
   int put_tiles(int si, const std::vector<TileJob> &tiles, TileJob **d, size_t *cap) {
     if (tiles.size() * sizeof(TileJob) > *cap) {
-      if (*d) (void)hipFree(*d);
-      *cap = std::max<size_t>(tiles.size() * sizeof(TileJob) * 2, 65536);
-      if (hipMalloc((void **)d, *cap) != hipSuccess) return set_error(ERR_NOMEM, "server: tiles");
+      dev_free(*d);
+      *d = nullptr;
+      *cap = 0;
+      const size_t want = std::max<size_t>(tiles.size() * sizeof(TileJob) * 2, 65536);
+      const int rc = dev_alloc((void **)d, want, "server: tiles");
+      if (rc) return rc;
+      *cap = want;
     }
     if (!tiles.empty() && hipMemcpyAsync(*d, tiles.data(), tiles.size() * sizeof(TileJob),
                                          hipMemcpyHostToDevice, stream[si]) != hipSuccess)
@@ -276,10 +306,13 @@ struct Server {
       pre2[si].push_back((int)tiles2.size());
     }
     if (desc.size() * sizeof(InstDesc<T>) > cap_inst[si]) {
-      if (d_inst[si]) (void)hipFree(d_inst[si]);
-      cap_inst[si] = std::max<size_t>(desc.size() * sizeof(InstDesc<T>) * 2, 4096);
-      if (hipMalloc(&d_inst[si], cap_inst[si]) != hipSuccess)
-        return set_error(ERR_NOMEM, "server: descriptors");
+      dev_free(d_inst[si]);
+      d_inst[si] = nullptr;
+      cap_inst[si] = 0;
+      const size_t want = std::max<size_t>(desc.size() * sizeof(InstDesc<T>) * 2, 4096);
+      const int rc = dev_alloc(&d_inst[si], want, "server: descriptors");
+      if (rc) return rc;
+      cap_inst[si] = want;
     }
     // tables are read by launches queued on this stream: copy in stream order
     if (!desc.empty() && hipMemcpyAsync(d_inst[si], desc.data(), desc.size() * sizeof(InstDesc<T>),
@@ -473,7 +506,13 @@ struct Server {
         }
         r.left -= taken;
         node_updates += (double)r.spec.nx * r.spec.ny * taken;
-        if (r.left == 0) {
+        if (r.cancel) {  // cancelled while live: its memory returns now
+          r.state = CANCELLED;
+          r.t_done = t1;
+          ++cancelled;
+          dirty = true;
+          arena.release(r.off, r.bytes);
+        } else if (r.left == 0) {
           r.state = DONE;
           r.t_done = t1;
           ++completed;
@@ -515,15 +554,15 @@ le2d_server *le2d_server_create(size_t hbm_budget, int max_live, int plain) {
     return nullptr;
   }
   Server *s = new Server(hbm_budget, plain ? 1 : max_live, plain);
-  if (hipMalloc((void **)&s->arena_dev, hbm_budget) != hipSuccess) {
-    le2d::set_error(le2d::ERR_NOMEM, "server: arena of %zu bytes", hbm_budget);
+  if (le2d::dev_alloc((void **)&s->arena_dev, hbm_budget, "server: arena")) {
     delete s;
     return nullptr;
   }
   for (int k = 0; k < 2; ++k)
     if (hipStreamCreateWithFlags(&s->stream[k], hipStreamNonBlocking) != hipSuccess) {
       le2d::set_error(le2d::ERR_DEVICE, "server: stream");
-      (void)hipFree(s->arena_dev);
+      if (k == 1) (void)hipStreamDestroy(s->stream[0]);
+      le2d::dev_free(s->arena_dev);
       delete s;
       return nullptr;
     }
@@ -547,6 +586,18 @@ int le2d_server_set_pair_window(le2d_server *h, int lanes) {
   std::lock_guard<std::mutex> lk(h->s->mu);
   h->s->pair_fb = lanes;
   h->s->dirty = true;
+  return le2d::OK;
+}
+
+int le2d_server_set_admission(le2d_server *h, int max_queue, int window, int max_pass) {
+  if (!h || max_queue < 0 || window < 1 || max_pass < 0)
+    return le2d::set_error(le2d::ERR_ARG, "server: admission %d %d %d", max_queue, window,
+                           max_pass);
+  std::lock_guard<std::mutex> lk(h->s->mu);
+  h->s->max_queue = max_queue;
+  h->s->window = window;
+  h->s->max_pass = max_pass;
+  h->s->cv_work.notify_one();
   return le2d::OK;
 }
 
@@ -575,6 +626,11 @@ long long le2d_server_submit(le2d_server *h, const le2d_request *q) {
                             s->budget);
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->failed) return -le2d::set_error(le2d::ERR_DEVICE, "%s", s->fail_msg);
+  if (s->max_queue > 0 && (int)s->queue.size() >= s->max_queue) {
+    ++s->rejected;
+    return -le2d::set_error(le2d::ERR_BUSY, "server: %d requests queued (the limit)",
+                            (int)s->queue.size());
+  }
   const long long id = s->next_id++;
   le2d::Req &r = s->reqs[id];
   r.spec = *q;
@@ -599,6 +655,8 @@ int le2d_server_wait(le2d_server *h, long long id, double timeout_s) {
   else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done))
     return le2d::set_error(le2d::ERR_STATE, "server: request %lld not done", id);
   if (s->failed) return le2d::set_error(le2d::ERR_DEVICE, "%s", s->fail_msg);
+  if (r->state == le2d::CANCELLED)
+    return le2d::set_error(le2d::ERR_STATE, "server: request %lld was cancelled", id);
   return le2d::OK;
 }
 
@@ -619,10 +677,13 @@ int le2d_server_result(le2d_server *h, long long id, void *aos) {
   Server *s = h->s;
   le2d::Req r;
   {
+    // claim the result: a second call (or a cancel) finds it RELEASING and
+    // fails, so the region is downloaded and released exactly once
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->reqs.find(id);
     if (it == s->reqs.end() || it->second.state != le2d::DONE || !it->second.spec.keep_result)
       return le2d::set_error(le2d::ERR_STATE, "server: no kept result for %lld", id);
+    it->second.state = le2d::RELEASING;
     r = it->second;
   }
   le2d::DevGrid g;
@@ -635,18 +696,59 @@ int le2d_server_result(le2d_server *h, long long id, void *aos) {
   g.buf[1] = s->buf(r, 1);
   g.cur = r.cur;
   hipStream_t st;
-  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
-    return le2d::set_error(le2d::ERR_DEVICE, "server: result stream");
-  g.stream = st;
-  int rc = le2d::grid_download(&g, aos, 0);
-  (void)hipStreamDestroy(st);
-  if (rc) return rc;
+  int rc = le2d::OK;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+    rc = le2d::set_error(le2d::ERR_DEVICE, "server: result stream");
+  } else {
+    g.stream = st;
+    rc = le2d::grid_download(&g, aos, 0);
+    (void)hipStreamDestroy(st);
+  }
   std::lock_guard<std::mutex> lk(s->mu);
   le2d::Req &rr = s->reqs[id];
+  if (rc) {  // the result stays kept; the call may be repeated
+    rr.state = le2d::DONE;
+    return rc;
+  }
   s->arena.release(rr.off, rr.bytes);
   rr.state = le2d::RELEASED;
   s->cv_work.notify_one();
   return le2d::OK;
+}
+
+int le2d_server_cancel(le2d_server *h, long long id) {
+  if (!h) return le2d::ERR_ARG;
+  Server *s = h->s;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return le2d::set_error(le2d::ERR_ARG, "server: no request %lld", id);
+  le2d::Req &r = it->second;
+  switch (r.state) {
+    case le2d::QUEUED:
+      s->queue.erase(std::find(s->queue.begin(), s->queue.end(), id));
+      r.state = le2d::CANCELLED;
+      r.t_done = le2d::now_s();
+      ++s->cancelled;
+      s->cv_done.notify_all();
+      s->cv_work.notify_one();
+      return le2d::OK;
+    case le2d::LIVE:
+      r.cancel = true;  // the engine drops it and frees its memory after the chunk
+      return le2d::OK;
+    case le2d::DONE:
+      if (r.spec.keep_result) {  // discard the kept result
+        s->arena.release(r.off, r.bytes);
+        r.state = le2d::CANCELLED;
+        ++s->cancelled;
+        s->cv_work.notify_one();
+        return le2d::OK;
+      }
+      break;
+    default:
+      break;
+  }
+  return le2d::set_error(le2d::ERR_STATE, "server: request %lld cannot be cancelled (state %d)",
+                         id, r.state);
 }
 
 int le2d_server_times(le2d_server *h, long long id, double *t_submit, double *t_admit,
@@ -658,7 +760,8 @@ int le2d_server_times(le2d_server *h, long long id, double *t_submit, double *t_
   *t_submit = it->second.t_submit;
   *t_admit = it->second.t_admit;
   *t_done = it->second.t_done;
-  return it->second.state >= le2d::DONE ? le2d::OK : le2d::ERR_STATE;
+  return it->second.state >= le2d::DONE && it->second.state != le2d::CANCELLED ? le2d::OK
+                                                                               : le2d::ERR_STATE;
 }
 
 int le2d_server_get_stats(le2d_server *h, le2d_server_stats *out) {
@@ -673,6 +776,8 @@ int le2d_server_get_stats(le2d_server *h, le2d_server_stats *out) {
   out->busy_seconds = s->busy_s;
   out->bytes_in_use = s->arena.used();
   out->peak_bytes = s->peak_bytes;
+  out->cancelled = s->cancelled;
+  out->rejected = s->rejected;
   return le2d::OK;
 }
 
@@ -688,12 +793,12 @@ void le2d_server_destroy(le2d_server *h) {
   if (s->th.joinable()) s->th.join();
   for (int k = 0; k < 2; ++k) {
     if (s->stream[k]) (void)hipStreamSynchronize(s->stream[k]);
-    if (s->d_inst[k]) (void)hipFree(s->d_inst[k]);
-    if (s->d_tiles[k]) (void)hipFree(s->d_tiles[k]);
-    if (s->d_tiles2[k]) (void)hipFree(s->d_tiles2[k]);
+    le2d::dev_free(s->d_inst[k]);
+    le2d::dev_free(s->d_tiles[k]);
+    le2d::dev_free(s->d_tiles2[k]);
     if (s->stream[k]) (void)hipStreamDestroy(s->stream[k]);
   }
-  if (s->arena_dev) (void)hipFree(s->arena_dev);
+  le2d::dev_free(s->arena_dev);
   delete s;
   delete h;
 }

@@ -29,6 +29,7 @@ def declared_functions():
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"static __inline__.*?\n}\n", "", text, flags=re.S)
+        text = re.sub(r"^typedef [^;{]*;", "", text, flags=re.M)  # function-pointer types
         for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b([a-z][a-z0-9_]*)\s*\([^;{]*\)\s*;", text,
                              flags=re.M):
             names.add(m.group(1))
@@ -52,7 +53,7 @@ def test_python_constants_match_header():
     text = open(os.path.join(ROOT, "include", "le2d.h")).read()
     defs = {m.group(1): int(m.group(2))
             for m in re.finditer(r"^#define LE2D_([A-Z_]+) (\d+)\s*$", text, flags=re.M)}
-    for name in ("ERR_ARG", "ERR_DEVICE", "ERR_NOMEM", "ERR_STATE", "ERR_BUDGET"):
+    for name in ("ERR_ARG", "ERR_DEVICE", "ERR_NOMEM", "ERR_STATE", "ERR_BUDGET", "ERR_BUSY"):
         assert getattr(le2d_amd, name) == defs[name], name
     for name in ("SYNC_WAIT", "SYNC_ASYNC", "SYNC_COHERENT"):
         assert getattr(le2d_amd, name) == defs[name], name
@@ -91,6 +92,34 @@ def test_c_header_layout_matches(tmp_path):
         subprocess.run(["gcc", flag, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                        check=True)
         assert subprocess.run([str(exe)], capture_output=True, text=True).stdout.strip() == expect
+
+
+def test_extension_structs_match_python(tmp_path):
+    """le2d_request, le2d_server_stats and le2d_view: the C layout of
+    include/le2d.h and the ctypes mirrors agree."""
+    src = tmp_path / "probe2.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "le2d.h"\n'
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(le2d_request),'
+        'offsetof(le2d_request,keep_result),sizeof(le2d_server_stats),'
+        'offsetof(le2d_server_stats,cancelled),offsetof(le2d_server_stats,rejected),'
+        'sizeof(le2d_view),offsetof(le2d_view,plane));return 0;}\n')
+    exe = tmp_path / "probe2"
+    subprocess.run(["gcc", "-DUSE_DOUBLE", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    R, S, V = le2d_amd.le2d_request, le2d_amd.le2d_server_stats, le2d_amd.le2d_view
+    assert got == [ctypes.sizeof(R), R.keep_result.offset, ctypes.sizeof(S), S.cancelled.offset,
+                   S.rejected.offset, ctypes.sizeof(V), V.plane.offset]
+
+
+def test_allocator_hook_arguments(lib):
+    """Both allocator functions or neither: one alone is refused; both NULL
+    restores hipMalloc / hipFree (no GPU needed)."""
+    fn = le2d_amd.ALLOC_FN(lambda p, n, c: 1)
+    assert lib.le2d_set_device_allocator(ctypes.cast(fn, ctypes.c_void_p), None, None) == \
+        le2d_amd.ERR_ARG
+    assert lib.le2d_set_device_allocator(None, None, None) == 0
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
