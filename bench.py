@@ -84,6 +84,42 @@ def time_runs(t, steps, count):
     return els
 
 
+def sustained(t, n, steps, el, seconds):
+    """Rate over at least `seconds` of back-to-back steps (an even count),
+    run right after the timed steps: the clock the device holds under
+    sustained load rather than in a short window."""
+    k = max(steps, int(np.ceil(seconds / max(el / steps, 1e-6))))
+    k += k & 1
+    t0 = time.perf_counter()
+    t.step_n(k)
+    t.wait()
+    dt = time.perf_counter() - t0
+    return {"steps": k, "seconds": round(dt, 3), "Gnode_per_s": round(n * n * k / dt / 1e9, 3)}
+
+
+def dense_report(le2d_amd, n, dtype, steps=200, seed=5):
+    """The same grid with all five fields random (uniform in [-1, 1]): the
+    kernels have no data-dependent path, so the rate matches the disk
+    workload's."""
+    rng = np.random.default_rng(seed)
+    with le2d_amd.Task(n, n, 0.25, (1.0, 1.0), (2.0, 1.0, 1.0), dtype=dtype) as t:
+        v = t.view()
+        for j in range(0, n, 512):  # fill in slabs (bounded host temporaries)
+            v[j:j + 512] = rng.uniform(-1.0, 1.0, size=v[j:j + 512].shape)
+        t.lib.le_mark_host_dirty(t.ref)
+        if os.environ.get("LE2D_TUNE") != "0" and t.get_step_pairs() > 0:
+            t.tune_step_pairs()
+        t.step_n(10)
+        t.wait()
+        t0 = time.perf_counter()
+        t.step_n(steps)
+        t.wait()
+        dt = time.perf_counter() - t0
+    return {"workload": f"{n}x{n}, all five fields uniform in [-1, 1] (seed {seed}), "
+                        f"{steps} steps through le_step_n",
+            "Gnode_per_s": round(n * n * steps / dt / 1e9, 3), "seconds": round(dt, 3)}
+
+
 def repeat_stats(n, steps, els):
     """Throughput of each back-to-back timed run: median, min and max."""
     v = sorted(n * n * steps / e / 1e9 for e in els)
@@ -91,14 +127,14 @@ def repeat_stats(n, steps, els):
             "max": round(v[-1], 3)}
 
 
-def measured_traffic(n, dtype, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary for this workload (profiles/traffic_<kernel>_<n>_<dtype>.json),
-    or None when no such measurement exists."""
+def measured_counters(n, dtype, kernel):
+    """The committed rocprofv3 counter summary of `kernel` on this workload
+    (profiles/traffic_<kernel>_<n>_<dtype>.json: HBM bytes per launch from
+    FETCH_SIZE / WRITE_SIZE, VALU issue from the SQ counters), or {}."""
     p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{n}_{dtype}.json")
     if not os.path.exists(p):
-        return None
-    return json.load(open(p)).get("hbm_bytes_per_launch")
+        return {}
+    return json.load(open(p))
 
 
 def roofline(n, es, ms_pair, ms_step, ms_x, ms_y, dtype_name):
@@ -111,7 +147,8 @@ def roofline(n, es, ms_pair, ms_step, ms_x, ms_y, dtype_name):
     # sizeof(real), times the node-updates per node one launch performs.
     alg = nodes * 20 * es * steps
     achieved = alg / (ms * 1e-3) / 1e9
-    traffic = measured_traffic(n, dtype_name, kernel)
+    counters = measured_counters(n, dtype_name, kernel)
+    traffic = counters.get("hbm_bytes_per_launch")
     # a fused launch moves each node's five values in and out once, however
     # many steps it takes: that is its compulsory traffic; `traffic` is what
     # rocprof counted
@@ -122,6 +159,16 @@ def roofline(n, es, ms_pair, ms_step, ms_x, ms_y, dtype_name):
             "traffic": traffic,
             "traffic_GBps": round(actual, 1) if actual else None,
             "traffic_frac": round(actual / PEAK_HBM_GBS, 4) if actual else None,
+            # the fraction of the VALU issue rate the kernel uses (SQ
+            # counters of the committed profile): what bounds it, since its
+            # actual traffic stays well below the HBM peak
+            "valu_issue_frac": counters.get("valu_issue_frac"),
+            "waves_per_simd": counters.get("waves_per_simd"),
+            "valu_insts_per_node_update": counters.get("valu_insts_per_node_update"),
+            "frac_basis": "achieved counts the two-sweep bytes of every node-update (160 B FP64, "
+                          "80 B FP32); a two-step launch moves each node once for two steps, so "
+                          "frac can exceed 1; traffic_frac (counted HBM bytes) and "
+                          "valu_issue_frac are at most 1",
             "compulsory_bytes_per_launch": compulsory,
             "algorithmic_bytes_per_launch": alg, "steps_per_launch": steps,
             "ms_per_launch": {"k_step2_xy": round(ms_pair, 4) if ms_pair else None,
@@ -323,20 +370,30 @@ def batch_report(le2d_amd, steps=500, plain_steps=50, count=256, shape=None):
                        f"and disk, {steps} steps timed batched, {plain_steps} per-instance"}
     r64, b64 = run_batch(le2d_amd, np.float64, insts, steps, plain_steps, shape)
     r32, b32 = run_batch(le2d_amd, np.float32, insts, steps, plain_steps, shape)
-    # both advanced the same number of steps from the same disks
+    # both advanced the same number of steps from the same disks: 4 warm-up,
+    # the batched and the per-instance timed steps (autotuning ran on the
+    # zero state before the disks were set)
     e2 = r2 = 0.0
-    einf = rinf = worst = 0.0
+    einf = rinf = 0.0
+    per = []
     for i in range(len(insts)):
         n = le2d_amd.diff_norms(b32.device_view(i), b64.device_view(i))
         e2 += n[0] ** 2
         r2 += n[1] ** 2
         einf, rinf = max(einf, n[2]), max(rinf, n[3])
-        worst = max(worst, n[0] / n[1] if n[1] > 0 else 0.0)
+        per.append(n[0] / n[1] if n[1] > 0 else 0.0)
     b64.close()
     b32.close()
+    per = np.array(per)
+    w = int(np.argmax(per))
     r32["l2_rel_vs_fp64"] = float(np.sqrt(e2 / r2))
-    r32["l2_rel_vs_fp64_worst_instance"] = worst
     r32["linf_rel_vs_fp64"] = einf / rinf
+    r32["l2_rel_vs_fp64_per_instance"] = {
+        "p50": float(np.percentile(per, 50)), "p90": float(np.percentile(per, 90)),
+        "p99": float(np.percentile(per, 99)), "max": float(per.max()),
+        "over_1e-5": int((per > 1e-5).sum())}
+    r32["worst_instance"] = dict(index=w, l2_rel=float(per[w]),
+                                 steps_from_disk=4 + steps + plain_steps, **insts[w])
     out["fp32"] = r32
     out["fp64"] = r64
     return out
@@ -346,7 +403,15 @@ def batch_report(le2d_amd, steps=500, plain_steps=50, count=256, shape=None):
 # Precision tier with two materials
 
 
-def tier_report(le2d_amd, n=16384, steps=2000, seed=7):
+def tier_report(le2d_amd, n=16384, steps=2000, seed=7, every=250):
+    """Two seeded materials split at x = n/2 and a disk of radius 64 placed
+    256 cells left of the split, so that the P front (0.5 cells per step in
+    the left material) reaches the interface after about 380 steps and the
+    transmitted and reflected waves run for the rest of the 2000.  FP64 and
+    FP32 runs from the same state; the energy is taken every `every` steps
+    outside the timed segments.  After the run the FP64 state is read back
+    once to show that the wave crossed: the largest |field| 200 cells right
+    of the interface."""
     rng = np.random.default_rng(seed)
     mats = []
     for _ in range(2):
@@ -355,11 +420,12 @@ def tier_report(le2d_amd, n=16384, steps=2000, seed=7):
     dt = 0.5 / max(m[0] for m in mats)
     ids = np.zeros((n, n), np.uint8)
     ids[:, n // 2:] = 1
-    cx, cy = disk_centre(n, seed)
+    cx, cy = float(n // 2 - 256), disk_centre(n, seed)[1]
     out = {"workload": f"{n}x{n}, two materials (c1, c2, rho) = "
                        f"{tuple(round(v, 4) for v in mats[0])} | "
                        f"{tuple(round(v, 4) for v in mats[1])} split at x = {n // 2}, "
-                       f"dt = {dt:.4f}, disk r = 64 at ({cx:.0f}, {cy:.0f}), {steps} steps"}
+                       f"dt = {dt:.4f}, disk r = 64 at ({cx:.0f}, {cy:.0f}), 256 cells left "
+                       f"of the interface, {steps} steps"}
     tasks = {}
     for name, dtype in (("fp64", np.float64), ("fp32", np.float32)):
         log(f"tier {name}")
@@ -368,17 +434,27 @@ def tier_report(le2d_amd, n=16384, steps=2000, seed=7):
         t.set_materials(mats, ids)
         e0 = t.energy()
         t.wait()
-        t0 = time.perf_counter()
-        t.step_n(steps)
-        t.wait()
-        el = time.perf_counter() - t0
+        curve, el, done = [1.0], 0.0, 0
+        while done < steps:
+            k = min(every, steps - done)
+            t0 = time.perf_counter()
+            t.step_n(k)
+            t.wait()
+            el += time.perf_counter() - t0
+            done += k
+            curve.append(round(t.energy() / e0, 6))
         e1 = t.energy()
         out[name] = {"Gnode_per_s": round(n * n * steps / el / 1e9, 3), "seconds": round(el, 3),
                      "energy_initial": e0, "energy_final": e1,
-                     "energy_rel_drift": (e1 - e0) / e0}
+                     "energy_rel_drift": (e1 - e0) / e0,
+                     f"energy_every_{every}_steps": curve}
         tasks[name] = t
     norms = le2d_amd.diff_norms(tasks["fp32"].device_view(), tasks["fp64"].device_view())
     out["fp32"].update({"vs_fp64_" + k: v for k, v in rel_errors(norms).items()})
+    out["fp32"]["energy_vs_fp64_rel"] = (out["fp32"]["energy_final"] / out["fp64"]["energy_final"]
+                                         - 1.0)
+    g = tasks["fp64"].grid()
+    out["fp64"]["peak_abs_200_cells_past_interface"] = float(np.abs(g[:, n // 2 + 200, :]).max())
     for t in tasks.values():
         t.close()
     return out
@@ -423,12 +499,39 @@ def drain_logged(s, n, what, limit_s):
         log(f"{what}: {s.stats()['completed']} of {n} requests done")
 
 
-def serve_report(le2d_amd, window_s=1.0, budget_gb=200.0, slots=1024):
+def serial_sample(reqs, every):
+    """A stratified sample of the requests for the serial baseline: within
+    each precision, the requests ranked by work (nodes x steps) and every
+    `every`-th of them taken, starting mid-stratum.  Returns, per sampled
+    request, its index and the indices of the stratum it stands for."""
+    out = []
+    for dt in (np.float32, np.float64):
+        ks = sorted((k for k, q in enumerate(reqs) if q["dtype"] == dt),
+                    key=lambda k: reqs[k]["nx"] * reqs[k]["ny"] * reqs[k]["steps"])
+        for a in range(0, len(ks), every):
+            stratum = ks[a:a + every]
+            out.append((stratum[len(stratum) // 2], stratum))
+    return out
+
+
+def serve_report(le2d_amd, window_s=60.0, budget_gb=200.0, slots=1024, chunk=16,
+                 serial_every=20, plain_rates=None):
+    """Continuous batching of a Poisson request stream against the serial
+    plain path.  The batched server receives the requests in real time.  The
+    serial path runs a stratified sample (one request in `serial_every`,
+    ranked by work within each precision) one at a time through the
+    single-grid kernels; each sampled request's service rate stands for its
+    stratum, and the serial queue is then replayed on the same arrivals.
+    `plain_rates` (FP32, FP64 Gnode-updates/s of one large grid) gives the
+    work-weighted ceiling of the mix."""
     reqs = serve_requests(window_s)
     work = sum(q["nx"] * q["ny"] * q["steps"] for q in reqs)
-    out = {"workload": f"Poisson 200 req/s for {window_s} s ({len(reqs)} requests, "
-                       f"{work / 1e9:.1f} Gnode-updates), sides log-uniform [256, 4096], steps "
-                       f"U[200, 2000], 70% FP32 / 30% FP64, HBM budget {budget_gb:.0f} GB"}
+    w32 = sum(q["nx"] * q["ny"] * q["steps"] for q in reqs if q["dtype"] == np.float32)
+    out = {"workload": f"Poisson 200 req/s for {window_s:g} s ({len(reqs)} requests, "
+                       f"{work / 1e9:.1f} Gnode-updates, {w32 / work:.2f} of it FP32), sides "
+                       f"log-uniform [256, 4096], steps U[200, 2000], 70% FP32 / 30% FP64 "
+                       f"requests, HBM budget {budget_gb:.0f} GB, {slots} slots, admission "
+                       f"every {chunk} steps"}
     budget = int(budget_gb * (1 << 30))
 
     # the FP32 results of a few (the cheapest) FP32 requests are kept from the
@@ -436,31 +539,29 @@ def serve_report(le2d_amd, window_s=1.0, budget_gb=200.0, slots=1024):
     f32 = [k for k, q in enumerate(reqs) if q["dtype"] == np.float32]
     probe = sorted(f32, key=lambda k: reqs[k]["nx"] * reqs[k]["ny"] * reqs[k]["steps"])[:4]
 
-    def submit_all(s, realtime, keep=()):
+    def submit(s, q, keep=False, dtype=None):
+        return s.submit(q["nx"], q["ny"], q["steps"], dtype or q["dtype"], q["dt"], (1.0, 1.0),
+                        q["mat"], q["disk"], keep=keep)
+
+    # continuous batching, arrivals in real time
+    log(f"serving {len(reqs)} requests over {window_s:g} s")
+    with le2d_amd.Server(budget, slots) as s:
+        s.set_shape(256, 16, chunk)
         ids = []
         t0 = time.perf_counter()
         for k, q in enumerate(reqs):
-            if realtime:
-                dt = q["at"] - (time.perf_counter() - t0)
-                if dt > 0:
-                    time.sleep(dt)
-            ids.append(s.submit(q["nx"], q["ny"], q["steps"], q["dtype"], q["dt"], (1.0, 1.0),
-                                q["mat"], q["disk"], keep=k in keep))
-        return ids
-
-    # continuous batching, arrivals in real time
-    log(f"serving {len(reqs)} requests")
-    with le2d_amd.Server(budget, slots) as s:
-        ids = submit_all(s, True, keep=probe)
-        drain_logged(s, len(reqs), "batched", 300 + 10 * window_s)
+            dt = q["at"] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(submit(s, q, keep=k in probe))
+        log(f"batched: all {len(reqs)} submitted")
+        drain_logged(s, len(reqs), "batched", 300 + 5 * window_s)
         times = [s.times(i) for i in ids]
         st = s.stats()
         mixed = {k: s.result(ids[k]) for k in probe}
     errs = []
     with le2d_amd.Server(budget, slots) as s:
-        twin = {k: s.submit(reqs[k]["nx"], reqs[k]["ny"], reqs[k]["steps"], np.float64,
-                            reqs[k]["dt"], (1.0, 1.0), reqs[k]["mat"], reqs[k]["disk"], keep=True)
-                for k in probe}
+        twin = {k: submit(s, reqs[k], keep=True, dtype=np.float64) for k in probe}
         s.drain(300)
         for k in probe:
             ref = s.result(twin[k])
@@ -472,32 +573,48 @@ def serve_report(le2d_amd, window_s=1.0, budget_gb=200.0, slots=1024):
     t_first = min(t[0] for t in times)
     span = max(t[2] for t in times) - t_first
     lat = [t[2] - t[0] for t in times]
-    out["batched"] = {"completed": st["completed"], "throughput_Gnode_per_s":
-                      round(st["node_updates"] / span / 1e9, 3),
+    served = st["node_updates"] / span / 1e9
+    out["batched"] = {"completed": st["completed"], "throughput_Gnode_per_s": round(served, 3),
                       "gpu_busy_Gnode_per_s": round(st["node_updates"] / st["busy_seconds"] / 1e9, 3),
                       "p50_latency_s": round(pct(lat, 50), 3), "p99_latency_s": round(pct(lat, 99), 3),
                       "makespan_s": round(span, 3), "peak_live": st["peak_live"],
                       "peak_GB": round(st["peak_bytes"] / 2 ** 30, 2)}
-    # serial plain path: one request at a time, arrival order; the service
-    # times are measured on the GPU, the queueing follows the same arrivals
-    log("serial baseline")
+    # serial plain path on a stratified sample: each sampled request's
+    # service time (initialisation and steps, the gap between consecutive
+    # completions of a one-slot server) scaled by work to its stratum
+    sample = serial_sample(reqs, serial_every)
+    log(f"serial baseline: {len(sample)} sampled requests")
     with le2d_amd.Server(budget, 1, plain=True) as s:
-        ids = submit_all(s, False)
-        drain_logged(s, len(reqs), "serial", 300 + 10 * window_s)
-        # the engine admits request k+1 when k completes: its service time
-        # (initialisation and steps) is the gap between completions
-        done = [s.times(i)[2] for i in ids]
-        svc = [done[0] - s.times(ids[0])[1]] + [b - a for a, b in zip(done, done[1:])]
+        sids = [submit(s, reqs[k]) for k, _ in sample]
+        drain_logged(s, len(sample), "serial", 300 + 5 * window_s)
+        done = [s.times(i)[2] for i in sids]
+        svc = [done[0] - s.times(sids[0])[1]] + [b - a for a, b in zip(done, done[1:])]
+    rate = {}
+    for (k, stratum), d in zip(sample, svc):
+        per = d / (reqs[k]["nx"] * reqs[k]["ny"] * reqs[k]["steps"])
+        for m in stratum:
+            rate[m] = per
     free, lat_s = 0.0, []
-    for q, d in zip(reqs, svc):
+    for m, q in enumerate(reqs):
         start = max(q["at"], free)
-        free = start + d
+        free = start + rate[m] * q["nx"] * q["ny"] * q["steps"]
         lat_s.append(free - q["at"])
-    out["serial"] = {"throughput_Gnode_per_s": round(work / (free - reqs[0]["at"]) / 1e9, 3),
+    serial = work / (free - reqs[0]["at"]) / 1e9
+    out["serial"] = {"throughput_Gnode_per_s": round(serial, 3),
                      "p50_latency_s": round(pct(lat_s, 50), 3),
-                     "p99_latency_s": round(pct(lat_s, 99), 3), "makespan_s": round(free, 3)}
-    out["throughput_ratio_batched_vs_serial"] = round(
-        out["batched"]["throughput_Gnode_per_s"] / out["serial"]["throughput_Gnode_per_s"], 3)
+                     "p99_latency_s": round(pct(lat_s, 99), 3), "makespan_s": round(free, 3),
+                     "method": f"{len(sample)} requests run one at a time (one in {serial_every}, "
+                               f"ranked by work within each precision); each one's service time "
+                               f"per node-update stands for its stratum; the queue is replayed "
+                               f"on the arrivals"}
+    out["throughput_ratio_batched_vs_serial"] = round(served / serial, 3)
+    if plain_rates:
+        r32, r64 = plain_rates
+        ceiling = work / 1e9 / (w32 / 1e9 / r32 + (work - w32) / 1e9 / r64)
+        out["blended_ceiling_Gnode_per_s"] = round(ceiling, 3)
+        out["ceiling_basis"] = (f"work-weighted: FP32 work at {r32:.1f} and FP64 work at "
+                                f"{r64:.1f} Gnode-updates/s, the plain path on one 8192^2 grid")
+        out["served_vs_ceiling"] = round(served / ceiling, 3)
     return out
 
 
@@ -517,8 +634,12 @@ def main():
     ap.add_argument("--no-ref", action="store_true", help="skip the 512x512 reference-case leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe) leg")
     ap.add_argument("--no-slab", action="store_true", help="skip the one-GPU-slab leg")
-    ap.add_argument("--serve-window", type=float, default=1.0,
+    ap.add_argument("--serve-window", type=float, default=60.0,
                     help="seconds of Poisson arrivals in the serving leg")
+    ap.add_argument("--serve-chunk", type=int, default=16,
+                    help="longest run of steps between the serving loop's admission points")
+    ap.add_argument("--serial-every", type=int, default=20,
+                    help="serial baseline: one request in this many (stratified by work)")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--batch-plain-steps", type=int, default=50,
                     help="steps of the one-launch-per-instance comparison loop")
@@ -526,6 +647,10 @@ def main():
                     help="two-step batch launches at this window and segment height "
                          "instead of autotuning (the same kernel in every profiling pass)")
     ap.add_argument("--tier-steps", type=int, default=2000)
+    ap.add_argument("--sustain", type=float, default=2.0,
+                    help="seconds of back-to-back steps after the timed ones (sustained rate)")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the all-fields-random leg")
     ap.add_argument("--repeats", type=int, default=5,
                     help="timed runs of --steps steps; value is the first, the median, "
                          "min and max of all are reported beside it")
@@ -568,9 +693,19 @@ def main():
         # the step count above
         other["repeats"] = repeat_stats(n, args.steps,
                                         els2 + time_runs(t2, args.steps, args.repeats - 1))
+        other["sustained"] = sustained(t2, n, args.steps, el2, args.sustain)
         t2.close()
+        if not args.no_dense:
+            log(f"{n}x{n} {o_name} dense field")
+            other["dense_field"] = dense_report(le2d_amd, n, o_dt)
     els += time_runs(t, args.steps, args.repeats - 1)
+    log(f"{n}x{n} {args.dtype}: sustained {args.sustain:g} s")
+    sus = sustained(t, n, args.steps, el, args.sustain)
     t.close()
+    dense = None
+    if not args.no_dense:
+        log(f"{n}x{n} {args.dtype} dense field")
+        dense = dense_report(le2d_amd, n, main_dt)
     copy_bytes = n * n * 5 * es
     copy_gbs = 2 * copy_bytes / le2d_amd.copy_probe(copy_bytes, 10) / 1e6
 
@@ -588,7 +723,12 @@ def main():
         tier = tier_report(le2d_amd, steps=args.tier_steps)
     serving = None
     if not args.no_serve:
-        serving = serve_report(le2d_amd, window_s=args.serve_window)
+        rates = None
+        if other is not None:
+            r64, r32 = (value, other["value"]) if main_dt == np.float64 else (other["value"], value)
+            rates = (r32, r64)
+        serving = serve_report(le2d_amd, window_s=args.serve_window, chunk=args.serve_chunk,
+                               serial_every=args.serial_every, plain_rates=rates)
     cpu = None
     if not args.no_cpu:
         log("CPU baseline")
@@ -606,6 +746,8 @@ def main():
                    "tiles": dict(zip(("x_rows", "y_seg", "y_block", "xy_seg", "xy_block",
                                       "pair_seg", "pair_lanes"), tiles))},
         "repeats": repeat_stats(n, args.steps, els),
+        "sustained": sus,
+        "dense_field": dense,
         "hbm_copy_GBps": round(copy_gbs, 1),
         "roofline": roof,
         "cpu_baseline": cpu,
