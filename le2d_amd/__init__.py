@@ -15,7 +15,9 @@ import numpy as np
 from .abi import ABI_D, ABI_F, ST_AOS, ST_SOA, NODE_FIELDS  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBDIR = os.path.join(HERE, "lib")
+# LE2D_LIBDIR selects another build of the same libraries (the host-sanitizer
+# build of le2d_amd/csrc `make san` lives in le2d_amd/lib_san).
+LIBDIR = os.environ.get("LE2D_LIBDIR") or os.path.join(HERE, "lib")
 
 SYNC_WAIT, SYNC_ASYNC, SYNC_COHERENT = 0, 1, 2
 # LE2D_ERR_* status codes of include/le2d.h

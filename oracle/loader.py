@@ -29,7 +29,9 @@ class Oracle:
         self.dtype = np.dtype(dtype)
         self.real = ctypes.c_double if self.dtype == np.float64 else ctypes.c_float
         name = "liboracle_d.so" if self.dtype == np.float64 else "liboracle_f.so"
-        path = os.path.join(HERE, name)
+        # LE2D_ORACLE_LIBDIR: another build of the oracle (oracle/_san, the
+        # sanitizer build of `make san`)
+        path = os.path.join(os.environ.get("LE2D_ORACLE_LIBDIR") or HERE, name)
         if not os.path.exists(path):
             build_oracle()
         self.lib = ctypes.CDLL(path)
