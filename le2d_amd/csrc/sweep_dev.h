@@ -38,7 +38,11 @@ __device__ __forceinline__ T superbee(T r) {
     // instructions.  A NaN r (0/0: both slopes exactly -eps) gives 0 here,
     // the first-order flux, where the reference's formula gives 2.
     const float c = __builtin_fminf(__builtin_fmaxf(r + r, 0.0f), 1.0f);
+#if LE2D_FP32_NAN2
+    return __builtin_fmaxf(c, __builtin_fminf(r, 2.0f));  // NaN: max(0, 2) = 2
+#else
     return __builtin_amdgcn_fmed3f(r, c, 2.0f);
+#endif
   } else {
     // FP64: max(c, min(r, 2)) is the reference's value bit for bit for every
     // r (NaN and infinities included: the clamp sends NaN to 0, min(NaN, 2)
@@ -65,9 +69,45 @@ __device__ __forceinline__ T superbee(T r) {
 // hardware reciprocal (about 1 ulp) times the numerator; that tier is held to
 // an error bound against FP64, not to bit parity, and the correctly rounded
 // FP32 division sequence costs several times more issue slots.
-__device__ __forceinline__ double slope_ratio(double n, double d) { return n / d; }
+// Build options of the FP32 tier's accuracy/cost trade-off (defaults in
+// brackets): LE2D_FP32_RATIO [0] the reciprocal times the numerator, 1 the
+// same refined by one Newton step on the quotient, 2 IEEE division;
+// LE2D_FP32_NAN2 [0] a NaN ratio gives the limiter 0, 1 gives 2 as the
+// reference's formula does.
+#ifndef LE2D_FP32_RATIO
+#define LE2D_FP32_RATIO 0
+#endif
+#ifndef LE2D_FP32_NAN2
+#define LE2D_FP32_NAN2 0
+#endif
+// LE2D_DIV_UNSCALED = 1 (measurement only, not a supported build): the FP64
+// division without its operand scaling, exact only while the operands'
+// exponents stay in the range where the scaling is an identity.
+#ifndef LE2D_DIV_UNSCALED
+#define LE2D_DIV_UNSCALED 0
+#endif
+__device__ __forceinline__ double slope_ratio(double n, double d) {
+#if LE2D_DIV_UNSCALED
+  double rc = __builtin_amdgcn_rcp(d);
+  rc = __builtin_fma(rc, __builtin_fma(-d, rc, 1.0), rc);
+  rc = __builtin_fma(rc, __builtin_fma(-d, rc, 1.0), rc);
+  double q = n * rc;
+  q = __builtin_fma(__builtin_fma(-d, q, n), rc, q);
+  return __builtin_amdgcn_div_fixup(q, d, n);
+#else
+  return n / d;
+#endif
+}
 __device__ __forceinline__ float slope_ratio(float n, float d) {
+#if LE2D_FP32_RATIO == 2
+  return n / d;
+#elif LE2D_FP32_RATIO == 1
+  const float rc = __builtin_amdgcn_rcpf(d);
+  const float q = n * rc;
+  return __builtin_fmaf(__builtin_fmaf(-d, q, n), rc, q);
+#else
   return n * __builtin_amdgcn_rcpf(d);
+#endif
 }
 
 template <typename T>
@@ -440,15 +480,27 @@ __device__ __forceinline__ void march_tile_alt(const T *__restrict__ src, T *__r
 // x-sweep of one window row with clamped neighbour lanes: the face between
 // lanes t-1 and t reads lanes nb = (t-2, t-1, t, t+1), each clamped into the
 // lanes holding grid columns.  Meaningful on lanes 2..FB-3.
+// LE2D_XS_NOBRANCH = 1: every lane forms a face and applies (lanes outside
+// 2..FB-3 produce values no output reads; their LDS indices stay in the
+// window), so the x-sweep has no divergent branch.
+#ifndef LE2D_XS_NOBRANCH
+#define LE2D_XS_NOBRANCH 0
+#endif
 template <typename T, int FB>
 __device__ __forceinline__ Node<T> window_xsweep_cl(Node<T> u, W4<T> *sw, W4<T> *sf, int t,
                                                     const int4 &nb, const SweepCoef<T> &c) {
   const W4<T> w = invariants(u.vx, u.sxx, u.vy, u.sxy, c);
   sw[t] = w;
   window_sync<FB>();
+#if LE2D_XS_NOBRANCH
+  sf[t] = faces(sw[nb.x], sw[nb.y], sw[nb.z], sw[nb.w], c);
+  window_sync<FB>();
+  apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, sf[t], sf[min(t + 1, FB - 1)], c);
+#else
   if (t >= 2 && t <= FB - 2) sf[t] = faces(sw[nb.x], sw[nb.y], sw[nb.z], sw[nb.w], c);
   window_sync<FB>();
   if (t >= 2 && t <= FB - 3) apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, sf[t], sf[t + 1], c);
+#endif
   return u;
 }
 
