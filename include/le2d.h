@@ -158,6 +158,15 @@ int le2d_task_view(le_task *task, le2d_view *v);
 /* out = {||a - ref||_2, ||ref||_2, max|a - ref|, max|ref|} over all five
  * fields, accumulated in FP64 on the device. */
 int le2d_diff_norms(const le2d_view *a, const le2d_view *ref, double out[4]);
+/* le2d_diff_norms without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under graph capture
+ * set-up, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-workgroup partials live in a fixed 32 KiB device array of the library
+ * (one per GPU, held until the process ends) instead of a buffer allocated
+ * per call; its size does not depend on the grid, so any grid is accepted.
+ * Results are bit-identical to le2d_diff_norms.  Calls that overlap on one
+ * GPU (from several threads) share that array and race: serialise them. */
+int le2d_diff_norms_noalloc(const le2d_view *a, const le2d_view *ref, double out[4]);
 
 /* ---------------------------------------------------------------------------
  * Batched instances: many independent grids of uneven sizes and homogeneous

@@ -106,6 +106,8 @@ _BATCH = {
     "le2d_batch_view": ([_VP, ctypes.c_int, ctypes.POINTER(le2d_view)], ctypes.c_int),
     "le2d_diff_norms": ([ctypes.POINTER(le2d_view), ctypes.POINTER(le2d_view), _DP],
                         ctypes.c_int),
+    "le2d_diff_norms_noalloc": ([ctypes.POINTER(le2d_view), ctypes.POINTER(le2d_view), _DP],
+                                ctypes.c_int),
     "le2d_server_create": ([ctypes.c_size_t, ctypes.c_int, ctypes.c_int], _VP),
     "le2d_server_set_shape": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "le2d_server_set_pairs": ([_VP, ctypes.c_int], ctypes.c_int),

@@ -162,6 +162,10 @@ int step2_xy_mat(DevGrid *g, MatPlane *m);
 // stride q*, element size e*) of the same nx x ny grid.
 int diff_norms(const void *a, int ea, long pa, long qa, const void *b, int eb, long pb, long qb,
                int nx, int ny, hipStream_t stream, double out[4]);
+// The same norms with no allocation: partials in a fixed device array of the
+// library (one per device; concurrent calls on one device race).
+int diff_norms_noalloc(const void *a, int ea, long pa, long qa, const void *b, int eb, long pb,
+                       long qb, int nx, int ny, double out[4]);
 
 // Sum over the grid of the elastic energy density in FP64 (per-node material
 // from the plane if active, else the given homogeneous material).

@@ -767,6 +767,17 @@ int le2d_diff_norms(const le2d_view *a, const le2d_view *ref, double out[4]) {
                           ref->plane, a->nx, a->ny, nullptr, out);
 }
 
+int le2d_diff_norms_noalloc(const le2d_view *a, const le2d_view *ref, double out[4]) {
+  if (!a || !ref || a->nx != ref->nx || a->ny != ref->ny)
+    return le2d::set_error(LE2D_ERR_ARG, "diff_norms_noalloc: views of different grids");
+  if ((a->elem != 4 && a->elem != 8) || (ref->elem != 4 && ref->elem != 8) || a->nx < 1 ||
+      a->ny < 1)
+    return le2d::set_error(LE2D_ERR_ARG, "diff_norms_noalloc: bad view");
+  (void)hipDeviceSynchronize();  // the views' owners may have work queued on their streams
+  return le2d::diff_norms_noalloc(a->base, a->elem, a->pitch, a->plane, ref->base, ref->elem,
+                                  ref->pitch, ref->plane, a->nx, a->ny, out);
+}
+
 int le2d_copy_probe(size_t bytes, int iters, double *ms) {
   return le2d::copy_probe(bytes, iters, ms);
 }

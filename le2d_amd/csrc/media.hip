@@ -9,6 +9,7 @@
 // same input buffer and write disjoint output tiles.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "sweep_dev.h"
@@ -149,9 +150,12 @@ __global__ __launch_bounds__(256) void k_diff_norms(const TA *__restrict__ a, lo
     for (int c = 0; c < 4; ++c) partial[4 * blockIdx.x + c] = red[c][0];
 }
 
+// Workgroups of the norm kernel (it strides over any grid).
+constexpr int kNormBlocks = 1024;
+
 int diff_norms(const void *a, int ea, long pa, long qa, const void *b, int eb, long pb, long qb,
                int nx, int ny, hipStream_t stream, double out[4]) {
-  const int blocks = 1024;
+  const int blocks = kNormBlocks;
   double *partial = nullptr;
   const int arc = dev_alloc((void **)&partial, 4 * blocks * sizeof(double), "diff_norms");
   if (arc) return arc;
@@ -186,6 +190,88 @@ int diff_norms(const void *a, int ea, long pa, long qa, const void *b, int eb, l
   out[1] = std::sqrt(r2);
   out[2] = sm;
   out[3] = rm;
+  return OK;
+}
+
+// ---------------------------------------------------------------------------
+// diff_norms without allocation.  The per-workgroup partials live in a
+// file-scope device array (one copy per device, held until the process
+// ends) instead of a hipMalloc'd buffer, and a second kernel sums them on
+// the device in the order the host loop of diff_norms uses, so the results
+// are bit-identical to diff_norms.  The array's size does not depend on the
+// grid (the norm kernel strides over any grid with a fixed workgroup count).
+
+constexpr int kNormScratch = 4 * kNormBlocks + 4;  // partials, then the four totals
+static __device__ __align__(256) double scratch[kNormScratch];
+
+// Sums of the partials in workgroup order (sum of squares) and their maxima:
+// one lane, the arithmetic of diff_norms' host loop.
+__global__ __launch_bounds__(64) void k_sum_partials(double *__restrict__ sc, int blocks) {
+  if (threadIdx.x != 0) return;
+  double s2 = 0, r2 = 0, sm = 0, rm = 0;
+  for (int k = 0; k < blocks; ++k) {
+    s2 += sc[4 * k];
+    r2 += sc[4 * k + 1];
+    sm = sm < sc[4 * k + 2] ? sc[4 * k + 2] : sm;  // std::max(sm, v)
+    rm = rm < sc[4 * k + 3] ? sc[4 * k + 3] : rm;
+  }
+  sc[4 * blocks] = s2;
+  sc[4 * blocks + 1] = r2;
+  sc[4 * blocks + 2] = sm;
+  sc[4 * blocks + 3] = rm;
+}
+
+// Device address of `scratch` on each device, looked up once per device
+// with that device current.  The host-side name `scratch` is only a handle:
+// kernels receive the looked-up address.
+static std::mutex g_scratch_mu;
+static double *g_scratch[64] = {};
+
+static int scratch_on(int dev, double **p) {
+  if (dev < 0 || dev >= 64) return set_error(ERR_DEVICE, "diff_norms_noalloc: device %d", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *q = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !q)
+      return set_error(ERR_DEVICE, "diff_norms_noalloc: scratch address: %s",
+                       hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double *>(q);
+  }
+  *p = g_scratch[dev];
+  return OK;
+}
+
+int diff_norms_noalloc(const void *a, int ea, long pa, long qa, const void *b, int eb, long pb,
+                       long qb, int nx, int ny, double out[4]) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return set_error(ERR_DEVICE, "diff_norms_noalloc: device");
+  double *sc = nullptr;
+  int rc = scratch_on(dev, &sc);
+  if (rc) return rc;
+  const dim3 g(kNormBlocks), bl(256);
+  if (ea == 8 && eb == 8)
+    hipLaunchKernelGGL((k_diff_norms<double, double>), g, bl, 0, nullptr, (const double *)a, pa,
+                       qa, (const double *)b, pb, qb, nx, ny, sc);
+  else if (ea == 4 && eb == 8)
+    hipLaunchKernelGGL((k_diff_norms<float, double>), g, bl, 0, nullptr, (const float *)a, pa, qa,
+                       (const double *)b, pb, qb, nx, ny, sc);
+  else if (ea == 8 && eb == 4)
+    hipLaunchKernelGGL((k_diff_norms<double, float>), g, bl, 0, nullptr, (const double *)a, pa, qa,
+                       (const float *)b, pb, qb, nx, ny, sc);
+  else
+    hipLaunchKernelGGL((k_diff_norms<float, float>), g, bl, 0, nullptr, (const float *)a, pa, qa,
+                       (const float *)b, pb, qb, nx, ny, sc);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, nullptr, sc, kNormBlocks);
+  double tot[4];
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpy(tot, sc + 4 * kNormBlocks, sizeof tot, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return set_error(ERR_DEVICE, "diff_norms_noalloc: %s", hipGetErrorString(e));
+  out[0] = std::sqrt(tot[0]);
+  out[1] = std::sqrt(tot[1]);
+  out[2] = tot[2];
+  out[3] = tot[3];
   return OK;
 }
 
