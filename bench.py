@@ -137,12 +137,14 @@ def measured_counters(n, dtype, kernel):
     return json.load(open(p))
 
 
-def roofline(n, es, ms_pair, ms_step, ms_x, ms_y, dtype_name):
+def roofline(n, es, ms_pair, ms_step, ms_x, ms_y, dtype_name, split=0):
     """Roofline of the launch that carries le_step_n: the two-step kernel
-    (two node-updates per node per launch) when it is on, else the
+    (two node-updates per node per launch; k_step2_split when the tuner
+    chose 64-lane windows on two wavefronts) when it is on, else the
     single-step kernel."""
     nodes = n * n
-    kernel, ms, steps = ("k_step2_xy", ms_pair, 2) if ms_pair else ("k_step_xy", ms_step, 1)
+    pair_kernel = "k_step2_split" if split else "k_step2_xy"
+    kernel, ms, steps = (pair_kernel, ms_pair, 2) if ms_pair else ("k_step_xy", ms_step, 1)
     # Algorithmic bytes per node-update: 2 sweeps x (5 reads + 5 writes) x
     # sizeof(real), times the node-updates per node one launch performs.
     alg = nodes * 20 * es * steps
@@ -171,7 +173,7 @@ def roofline(n, es, ms_pair, ms_step, ms_x, ms_y, dtype_name):
                           "valu_issue_frac are at most 1",
             "compulsory_bytes_per_launch": compulsory,
             "algorithmic_bytes_per_launch": alg, "steps_per_launch": steps,
-            "ms_per_launch": {"k_step2_xy": round(ms_pair, 4) if ms_pair else None,
+            "ms_per_launch": {pair_kernel: round(ms_pair, 4) if ms_pair else None,
                               "k_step_xy": round(ms_step, 4), "k_sweep_x": round(ms_x, 4),
                               "k_sweep_y": round(ms_y, 4)},
             "two_kernel_GBps": {"k_sweep_x": round(nodes * 10 * es / ms_x / 1e6, 1),
@@ -354,7 +356,9 @@ def run_batch(le2d_amd, dtype, insts, steps, plain_steps, shape=None):
            "shape": {"window_lanes": fb, "segment_rows": seg,
                      "launch": {"tiles": "one workgroup per tile, one step",
                                 "persistent": "persistent, one step",
-                                "pairs": "one workgroup per tile, two steps"}[b.form()]},
+                                "pairs": "one workgroup per tile, two steps",
+                                "pairs_split": "one workgroup per tile, two steps, the two "
+                                               "stages on two wavefronts"}[b.form()]},
            "batched_Gnode_per_s": round(nodes / ms_batch / 1e6, 3),
            "plain_Gnode_per_s": round(nodes / ms_plain / 1e6, 3),
            "speedup_vs_plain": round(ms_plain / ms_batch, 3),
@@ -515,7 +519,7 @@ def serial_sample(reqs, every):
 
 
 def serve_report(le2d_amd, window_s=60.0, budget_gb=200.0, slots=1024, chunk=16,
-                 serial_every=20, plain_rates=None):
+                 serial_every=20, plain_rates=None, split=0, serial=True):
     """Continuous batching of a Poisson request stream against the serial
     plain path.  The batched server receives the requests in real time.  The
     serial path runs a stratified sample (one request in `serial_every`,
@@ -547,6 +551,7 @@ def serve_report(le2d_amd, window_s=60.0, budget_gb=200.0, slots=1024, chunk=16,
     log(f"serving {len(reqs)} requests over {window_s:g} s")
     with le2d_amd.Server(budget, slots) as s:
         s.set_shape(256, 16, chunk)
+        s.set_pair_split(split)
         ids = []
         t0 = time.perf_counter()
         for k, q in enumerate(reqs):
@@ -579,6 +584,8 @@ def serve_report(le2d_amd, window_s=60.0, budget_gb=200.0, slots=1024, chunk=16,
                       "p50_latency_s": round(pct(lat, 50), 3), "p99_latency_s": round(pct(lat, 99), 3),
                       "makespan_s": round(span, 3), "peak_live": st["peak_live"],
                       "peak_GB": round(st["peak_bytes"] / 2 ** 30, 2)}
+    if not serial:
+        return out
     # serial plain path on a stratified sample: each sampled request's
     # service time (initialisation and steps, the gap between consecutive
     # completions of a one-slot server) scaled by work to its stratum
@@ -638,6 +645,10 @@ def main():
                     help="seconds of Poisson arrivals in the serving leg")
     ap.add_argument("--serve-chunk", type=int, default=16,
                     help="longest run of steps between the serving loop's admission points")
+    ap.add_argument("--serve-split", type=int, default=0, choices=(0, 1),
+                    help="serving: 64-lane two-step windows on two wavefronts")
+    ap.add_argument("--no-serial", action="store_true",
+                    help="serving: skip the serial baseline")
     ap.add_argument("--serial-every", type=int, default=20,
                     help="serial baseline: one request in this many (stratified by work)")
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -672,8 +683,9 @@ def main():
                                                      args.warmup, args.profile_steps)
     el = els[0]
     value = n * n * args.steps / el / 1e9
-    roof = roofline(n, es, ms_pair, ms_step, ms_x, ms_y, args.dtype)
-    tiles = t.get_tiles() + (t.get_step_pairs(), t.get_step_pair_window())
+    roof = roofline(n, es, ms_pair, ms_step, ms_x, ms_y, args.dtype, t.get_step_pair_split())
+    tiles = t.get_tiles() + (t.get_step_pairs(), t.get_step_pair_window(),
+                             t.get_step_pair_split())
     other = None
     if not args.no_fp32:
         o_dt = np.float32 if main_dt == np.float64 else np.float64
@@ -686,7 +698,11 @@ def main():
         errs = rel_errors(le2d_amd.diff_norms(lo.device_view(), hi.device_view()))
         other = {"dtype": o_name, "value": round(n * n * args.steps / el2 / 1e9, 3),
                  "ms_per_step": round(el2 / args.steps * 1e3, 4),
-                 "roofline": roofline(n, np.dtype(o_dt).itemsize, mp2, ms2, mx2, my2, o_name),
+                 "roofline": roofline(n, np.dtype(o_dt).itemsize, mp2, ms2, mx2, my2, o_name,
+                                      t2.get_step_pair_split()),
+                 "tiles": {"pair_seg": t2.get_step_pairs(),
+                           "pair_lanes": t2.get_step_pair_window(),
+                           "pair_split": t2.get_step_pair_split()},
                  "fp32_vs_fp64_after_steps": args.warmup + args.steps + 4 * args.profile_steps,
                  "fp32_vs_fp64": errs}
         # the further timed runs come after the comparison, which stays at
@@ -728,7 +744,8 @@ def main():
             r64, r32 = (value, other["value"]) if main_dt == np.float64 else (other["value"], value)
             rates = (r32, r64)
         serving = serve_report(le2d_amd, window_s=args.serve_window, chunk=args.serve_chunk,
-                               serial_every=args.serial_every, plain_rates=rates)
+                               serial_every=args.serial_every, plain_rates=rates,
+                               split=args.serve_split, serial=not args.no_serial)
     cpu = None
     if not args.no_cpu:
         log("CPU baseline")
@@ -744,7 +761,7 @@ def main():
                                f"({cx:.0f},{cy:.0f}), c1=2 c2=1 rho=1, dt=0.25",
                    "nx": n, "ny": n,
                    "tiles": dict(zip(("x_rows", "y_seg", "y_block", "xy_seg", "xy_block",
-                                      "pair_seg", "pair_lanes"), tiles))},
+                                      "pair_seg", "pair_lanes", "pair_split"), tiles))},
         "repeats": repeat_stats(n, args.steps, els),
         "sustained": sus,
         "dense_field": dense,

@@ -90,9 +90,16 @@ int le2d_get_step_pairs(le_task *task, int *seg);
  * of about 8192^2 and more. */
 int le2d_set_step_pair_window(le_task *task, int lanes);
 int le2d_get_step_pair_window(le_task *task, int *lanes);
+/* With 64-lane windows: on = 1 runs the two stages of a pass on two
+ * wavefronts of a 128-lane workgroup (each holds one stage's state, so more
+ * are resident), 0 on one wavefront.  Bit-identical either way; the tuner
+ * chooses between them with the window width. */
+int le2d_set_step_pair_split(le_task *task, int on);
+int le2d_get_step_pair_split(le_task *task, int *on);
 /* Time the two-step pass at segment heights 16 .. 160 (4 .. 160 on grids too
- * small to fill the device with 16-row segments) and windows of 64, 128 and
- * 256 lanes on the task's grid and keep the fastest.  The state does not
+ * small to fill the device with 16-row segments) and windows of 64 (one or
+ * two wavefronts, le2d_set_step_pair_split), 128 and 256 lanes on the task's
+ * grid and keep the fastest.  The state does not
  * advance: each candidate reads the current state and writes the idle
  * buffer.  le_step_n does this once per
  * grid shape and process on its first call of 8 or more steps, unless the
@@ -207,10 +214,12 @@ int le2d_batch_wait(le2d_batch *b);
  * LE2D_BATCH_TILES, one workgroup per tile; LE2D_BATCH_PERSISTENT, resident
  * workgroups walk the tile table; LE2D_BATCH_PAIRS, one workgroup per tile
  * and two steps per launch (an odd step count ends with one single-step
- * launch).  All forms give the same results. */
+ * launch); LE2D_BATCH_PAIRS_SPLIT, the same with 64-lane windows whose two
+ * stages run on two wavefronts.  All forms give the same results. */
 #define LE2D_BATCH_TILES 0
 #define LE2D_BATCH_PERSISTENT 1
 #define LE2D_BATCH_PAIRS 2
+#define LE2D_BATCH_PAIRS_SPLIT 3
 int le2d_batch_set_shape(le2d_batch *b, int fb, int seg);
 int le2d_batch_set_launch(le2d_batch *b, int fb, int seg, int form);
 int le2d_batch_get_launch(le2d_batch *b, int *form);
@@ -267,6 +276,9 @@ int le2d_server_set_pairs(le2d_server *srv, int pair_seg);
 /* Lanes per two-step workgroup, 64, 128 or 256; 0 (the default): 256 for
  * FP32 requests and 64 for FP64.  Results do not change. */
 int le2d_server_set_pair_window(le2d_server *srv, int lanes);
+/* With 64-lane two-step windows, on = 1 runs each pass's two stages on two
+ * wavefronts (default 0).  Results do not change. */
+int le2d_server_set_pair_split(le2d_server *srv, int on);
 /* Admission.  max_queue: requests that may wait (0, the default: no limit);
  * at the limit le2d_server_submit returns -LE2D_ERR_BUSY and the caller
  * retries later (back-pressure).  Admission takes queued requests in

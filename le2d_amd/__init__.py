@@ -40,6 +40,8 @@ _EXT = {
     "le2d_get_step_pairs": (["T", ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "le2d_set_step_pair_window": (["T", ctypes.c_int], ctypes.c_int),
     "le2d_get_step_pair_window": (["T", ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "le2d_set_step_pair_split": (["T", ctypes.c_int], ctypes.c_int),
+    "le2d_get_step_pair_split": (["T", ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "le2d_tune_step_pairs": (["T", ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "le2d_time_pairs": (["T", ctypes.c_int, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "le2d_copy_probe": ([ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)],
@@ -112,6 +114,7 @@ _BATCH = {
     "le2d_server_set_shape": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "le2d_server_set_pairs": ([_VP, ctypes.c_int], ctypes.c_int),
     "le2d_server_set_pair_window": ([_VP, ctypes.c_int], ctypes.c_int),
+    "le2d_server_set_pair_split": ([_VP, ctypes.c_int], ctypes.c_int),
     "le2d_server_set_admission": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
                                   ctypes.c_int),
     "le2d_server_submit": ([_VP, ctypes.POINTER(le2d_request)], ctypes.c_longlong),
@@ -311,6 +314,16 @@ class Task:
                     "get_step_pair_window")
         return v.value
 
+    def set_step_pair_split(self, on):
+        """64-lane windows: the two stages on two wavefronts (1) or one (0)."""
+        self._check(self.lib.le2d_set_step_pair_split(self.ref, int(on)), "set_step_pair_split")
+
+    def get_step_pair_split(self):
+        v = ctypes.c_int()
+        self._check(self.lib.le2d_get_step_pair_split(self.ref, ctypes.byref(v)),
+                    "get_step_pair_split")
+        return v.value
+
     def tune_step_pairs(self):
         """Time the two-step pass at several segment heights and both window
         widths and keep the fastest (the state does not advance); returns the
@@ -432,11 +445,12 @@ class Batch:
     def wait(self):
         self._check(self.lib.le2d_batch_wait(self.h), "le2d_batch_wait")
 
-    FORMS = {"tiles": 0, "persistent": 1, "pairs": 2}
+    FORMS = {"tiles": 0, "persistent": 1, "pairs": 2, "pairs_split": 3}
 
     def set_shape(self, fb, seg, form=None):
-        """Window width and rows per segment; form ("tiles", "persistent" or
-        "pairs") also picks the launch form (None keeps it)."""
+        """Window width and rows per segment; form ("tiles", "persistent",
+        "pairs" or "pairs_split") also picks the launch form (None keeps
+        it)."""
         if form is None:
             self._check(self.lib.le2d_batch_set_shape(self.h, fb, seg), "le2d_batch_set_shape")
         else:
@@ -444,7 +458,7 @@ class Batch:
                         "le2d_batch_set_launch")
 
     def form(self):
-        """The launch form: "tiles", "persistent" or "pairs"."""
+        """The launch form: "tiles", "persistent", "pairs" or "pairs_split"."""
         v = ctypes.c_int()
         self._check(self.lib.le2d_batch_get_launch(self.h, ctypes.byref(v)), "get_launch")
         return {n: k for k, n in self.FORMS.items()}[v.value]
@@ -518,6 +532,10 @@ class Server:
         """Lanes per two-step workgroup: 64, 128 or 256 (0: 256 for FP32
         requests, 64 for FP64)."""
         self._check(self.lib.le2d_server_set_pair_window(self.h, lanes), "set_pair_window")
+
+    def set_pair_split(self, on=1):
+        """64-lane two-step windows: the two stages on two wavefronts."""
+        self._check(self.lib.le2d_server_set_pair_split(self.h, int(on)), "set_pair_split")
 
     def set_admission(self, max_queue=0, window=64, max_pass=16):
         """Queue limit (0: none; submit then raises BusyError at the limit),

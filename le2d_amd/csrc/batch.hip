@@ -41,12 +41,15 @@ struct Inst {
 };
 
 // Launch shape: window width, rows per segment, and the launch form.
-enum { FORM_TILES = 0, FORM_PERSISTENT = 1, FORM_PAIRS = 2 };
+enum { FORM_TILES = 0, FORM_PERSISTENT = 1, FORM_PAIRS = 2, FORM_PAIRS_SPLIT = 3 };
+inline bool two_step(int form) { return form == FORM_PAIRS || form == FORM_PAIRS_SPLIT; }
 struct Shape {
   int fb, seg;
   int form;  // FORM_TILES: one workgroup per tile; FORM_PERSISTENT: resident
              // workgroups walk the table; FORM_PAIRS: one workgroup per tile of
-             // the two-step table, two steps per launch (fb 128 or 256)
+             // the two-step table, two steps per launch (fb 64, 128 or 256);
+             // FORM_PAIRS_SPLIT: the same with 64-lane windows whose two
+             // stages run on two wavefronts (march_pair_split)
 };
 
 std::mutex g_tune_mu;
@@ -172,9 +175,10 @@ struct BatchImpl {
 
   int build_tiles(Shape s) {
     if (s.form == FORM_PAIRS && s.fb != 64 && s.fb != 128) s.fb = 256;
-    int rc = upload_table(s.form == FORM_PAIRS ? s.fb - 8 : s.fb - 4, s.seg, &d_tiles, &ntiles);
+    if (s.form == FORM_PAIRS_SPLIT) s.fb = 64;
+    int rc = upload_table(two_step(s.form) ? s.fb - 8 : s.fb - 4, s.seg, &d_tiles, &ntiles);
     if (rc) return rc;
-    if (s.form == FORM_PAIRS) {
+    if (two_step(s.form)) {
       rc = upload_table(s.fb - 4, s.seg, &d_tiles1, &ntiles1);
       if (rc) return rc;
     }
@@ -202,6 +206,9 @@ struct BatchImpl {
     if (shape.form == FORM_PERSISTENT)
       hipLaunchKernelGGL((k_step_batch<T, FB>), dim3(blocks), dim3(FB), 0, stream, d, d_tiles,
                          ntiles, par);
+    else if (shape.form == FORM_PAIRS_SPLIT)
+      hipLaunchKernelGGL((k_step_batch_pairs_split<T>), dim3(ntiles), dim3(128), 0, stream, d,
+                         d_tiles, par);
     else if (shape.form == FORM_PAIRS)
       hipLaunchKernelGGL((k_step_batch_pairs<T, FB>), dim3(ntiles), dim3(FB), 0, stream, d,
                          d_tiles, par);
@@ -257,7 +264,7 @@ struct BatchImpl {
 
   int step(int steps, bool use_graph) {
     if (!committed) return set_error(ERR_STATE, "batch: not committed");
-    const int spl = shape.form == FORM_PAIRS ? 2 : 1;  // steps per launch
+    const int spl = two_step(shape.form) ? 2 : 1;  // steps per launch
     int k = 0;
     if (use_graph && steps >= 2 * spl) {
       int rc = ensure_graph(parity);  // two launches: the parity comes back
@@ -468,7 +475,7 @@ int le2d_batch_wait(le2d_batch *b) {
 
 int le2d_batch_set_launch(le2d_batch *b, int fb, int seg, int form) {
   if (!b || !b->impl.committed || (fb != 64 && fb != 128 && fb != 256) || seg < 1 ||
-      form < le2d::FORM_TILES || form > le2d::FORM_PAIRS)
+      form < le2d::FORM_TILES || form > le2d::FORM_PAIRS_SPLIT)
     return le2d::set_error(le2d::ERR_ARG, "batch: shape %d %d form %d", fb, seg, form);
   return b->impl.build_tiles({fb, seg, form});
 }
@@ -531,12 +538,13 @@ int le2d_batch_autotune(le2d_batch *b, int trials, int *best_fb, int *best_seg, 
       for (int seg : segs) cand.push_back({fb, seg, form});
   for (int fb : fbs)
     for (int seg : segs) cand.push_back({fb, seg, le2d::FORM_PAIRS});
+  for (int seg : segs) cand.push_back({64, seg, le2d::FORM_PAIRS_SPLIT});
   le2d::Shape best{B.shape};
   double best_t = 1e30;
   for (const le2d::Shape &c : cand) {
     int rc = B.build_tiles(c);
     if (rc) return rc;
-    const int spl = c.form == le2d::FORM_PAIRS ? 2 : 1;
+    const int spl = le2d::two_step(c.form) ? 2 : 1;
     rc = B.step(2 * spl, true);  // captures the graph, warms up
     if (rc) return rc;
     double ms = 0;

@@ -37,6 +37,7 @@ struct Tiles {
   int xy_block; // lanes per fused-step workgroup (64, 128, 256); writes xy_block-4 columns
   int pair_seg; // rows per segment of the two-step pass (0: one step per pass)
   int pair_block; // lanes per two-step workgroup (64, 128 or 256); writes pair_block-8 columns
+  int pair_split = 0; // 64-lane two-step windows: the two stages on two wavefronts
 };
 
 struct DevGrid {

@@ -275,6 +275,7 @@ static int best_segment(long strips, int ny, long resident, int halo, int cap);
 Tiles default_tiles(int elem, int nx, int ny) {
   Tiles t;
   t.x_rows = 4;
+  t.pair_split = 0;
   {
     // fused step: 256-lane windows (252 output columns) and ~64 workgroups
     // per CU in segments of at least 16 rows (measured best on 8192^2: 16-24

@@ -53,7 +53,7 @@ std::mutex g_mu;
 std::unordered_map<const real *, TaskState *> g_tasks;
 // tuned two-step segment heights per (element size, nx, ny), for the process
 std::mutex g_pair_mu;
-std::map<std::tuple<int, int, int>, std::pair<int, int>> g_pair_seg;  // (rows, lanes)
+std::map<std::tuple<int, int, int>, std::tuple<int, int, int>> g_pair_seg;  // rows, lanes, split
 
 int env_mode() {
   const char *s = getenv("LE2D_SYNC");
@@ -165,8 +165,9 @@ int tune_pairs(le_task *t, TaskState *s) {
     std::lock_guard<std::mutex> lk(g_pair_mu);
     auto it = g_pair_seg.find(key);
     if (it != g_pair_seg.end()) {
-      s->g.tiles.pair_seg = it->second.first;
-      s->g.tiles.pair_block = it->second.second;
+      s->g.tiles.pair_seg = std::get<0>(it->second);
+      s->g.tiles.pair_block = std::get<1>(it->second);
+      s->g.tiles.pair_split = std::get<2>(it->second);
       s->pairs_fixed = true;
       return LE2D_OK;
     }
@@ -177,9 +178,10 @@ int tune_pairs(le_task *t, TaskState *s) {
   for (auto &e : ev) (void)hipEventCreate(&e);
   int rc = LE2D_OK;
   // mean ms of one pass at (block, seg) after one untimed pass; < 0 on error
-  auto time_one = [&](int block, int seg, int reps) -> float {
+  auto time_one = [&](int block, int split, int seg, int reps) -> float {
     s->g.tiles.pair_seg = seg;
     s->g.tiles.pair_block = block;
+    s->g.tiles.pair_split = split;
     for (int k = 0; k <= reps && !rc; ++k) {
       if (k == 1) (void)hipEventRecord(ev[0], s->stream);
       rc = le2d::step2_xy<real>(&s->g, cx, cy);
@@ -196,20 +198,24 @@ int tune_pairs(le_task *t, TaskState *s) {
   };
   struct Cand {
     float ms;
-    int block, seg;
+    int block, split, seg;
   };
   // warm-up at the default shape, about 25 ms: after host-side work the
   // device clocks are still low and would make the first candidates look slow
   {
-    const int b0 = s->g.tiles.pair_block, s0 = s->g.tiles.pair_seg;
-    const float est = time_one(b0, s0, 1);
+    const int b0 = s->g.tiles.pair_block, p0 = s->g.tiles.pair_split, s0 = s->g.tiles.pair_seg;
+    const float est = time_one(b0, p0, s0, 1);
     if (est > 0) {
       const int reps = std::max(1, std::min(64, (int)(25.0f / est)));
-      (void)time_one(b0, s0, reps);
+      (void)time_one(b0, p0, s0, reps);
     }
   }
   std::vector<Cand> res;
-  for (int block : {256, 128, 64}) {
+  // windows: 256, 128 and 64 lanes, and 64-lane windows with the two stages
+  // on two wavefronts (k_step2_split)
+  const int blocks[4][2] = {{256, 0}, {128, 0}, {64, 0}, {64, 1}};
+  for (const auto &bs : blocks) {
+    const int block = bs[0], split = bs[1];
     // segments under 16 rows only for grids that 16-row segments leave short
     // of one round of resident workgroups (there a pass lasts one
     // workgroup's march)
@@ -219,28 +225,29 @@ int tune_pairs(le_task *t, TaskState *s) {
     for (int seg : cand) {
       if (seg < 16 && !small) continue;
       if (seg > s->g.ny && seg != cand[0]) break;  // taller segments are all one
-      const float ms = time_one(block, seg, 4);
+      const float ms = time_one(block, split, seg, 4);
       if (ms < 0) break;
-      res.push_back({ms, block, seg});
+      res.push_back({ms, block, split, seg});
     }
   }
-  Cand best{1e30f, 256, 64};
+  Cand best{1e30f, 256, 0, 64};
   if (!rc && !res.empty()) {
     std::sort(res.begin(), res.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
     if (res.size() > 3) res.resize(3);
     for (const Cand &c : res) {
-      const float ms = time_one(c.block, c.seg, 12);
+      const float ms = time_one(c.block, c.split, c.seg, 12);
       if (ms < 0) break;
-      if (ms < best.ms) best = {ms, c.block, c.seg};
+      if (ms < best.ms) best = {ms, c.block, c.split, c.seg};
     }
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
   s->g.tiles.pair_seg = best.seg;
   s->g.tiles.pair_block = best.block;
+  s->g.tiles.pair_split = best.split;
   s->pairs_fixed = true;
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(g_pair_mu);
-  g_pair_seg[key] = {best.seg, best.block};
+  g_pair_seg[key] = std::make_tuple(best.seg, best.block, best.split);
   return LE2D_OK;
 }
 
@@ -674,6 +681,24 @@ int le2d_set_step_pair_window(le_task *task, int lanes) {
     return le2d::set_error(LE2D_ERR_ARG, "step pairs: window %d", lanes);
   s->g.tiles.pair_block = lanes;
   s->pairs_fixed = true;
+  return LE2D_OK;
+}
+
+int le2d_set_step_pair_split(le_task *task, int on) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  if (on != 0 && on != 1) return le2d::set_error(LE2D_ERR_ARG, "step pairs: split %d", on);
+  s->g.tiles.pair_split = on;
+  s->pairs_fixed = true;
+  return LE2D_OK;
+}
+
+int le2d_get_step_pair_split(le_task *task, int *on) {
+  TaskState *s = find(task);
+  int rc = ensure_device(task, s);
+  if (rc) return rc;
+  *on = s->g.tiles.pair_split;
   return LE2D_OK;
 }
 

@@ -123,12 +123,14 @@ struct Server {
   // which are issue-bound, 64 (one wavefront, no barriers between waves)
   // for FP64, which are latency-bound
   int pair_fb = 0;
+  // 64-lane two-step windows with the two stages on two wavefronts
+  int pair_split = 0;
   // The settings above are written by API calls (under mu); the engine
   // thread launches with this copy, taken under mu whenever it rebuilds the
   // tables, so a setting that changes mid-chunk never meets tables built
   // for another.
   struct Launch {
-    int fb = 256, seg = 16, pair_seg = 64, pair_fb = 0;
+    int fb = 256, seg = 16, pair_seg = 64, pair_fb = 0, pair_split = 0;
   } act;
   int pair_lanes(int si) const { return act.pair_fb ? act.pair_fb : (si == 1 ? 64 : 256); }
   char *arena_dev = nullptr;
@@ -353,7 +355,10 @@ struct Server {
     const dim3 grid((unsigned)count);
     const InstDesc<T> *d = static_cast<const InstDesc<T> *>(d_inst[si]);
     const int lanes = pair_lanes(si);
-    if (lanes == 64)
+    if (lanes == 64 && act.pair_split)
+      hipLaunchKernelGGL((k_step_batch_pairs_split<T>), grid, dim3(128), 0, stream[si], d,
+                         d_tiles2[si], step_parity);
+    else if (lanes == 64)
       hipLaunchKernelGGL((k_step_batch_pairs<T, 64>), grid, dim3(64), 0, stream[si], d,
                          d_tiles2[si], step_parity);
     else if (lanes == 128)
@@ -452,7 +457,7 @@ struct Server {
           set[r.spec.real_size == 8 ? 1 : 0].push_back(&r);
         }
         if (dirty) {
-          act = {fb, seg, pair_seg, pair_fb};
+          act = {fb, seg, pair_seg, pair_fb, pair_split};
           rebuild = true;
           dirty = false;
         }
@@ -585,6 +590,14 @@ int le2d_server_set_pair_window(le2d_server *h, int lanes) {
     return le2d::set_error(le2d::ERR_ARG, "server: pair window %d", lanes);
   std::lock_guard<std::mutex> lk(h->s->mu);
   h->s->pair_fb = lanes;
+  h->s->dirty = true;
+  return le2d::OK;
+}
+
+int le2d_server_set_pair_split(le2d_server *h, int on) {
+  if (!h || (on != 0 && on != 1)) return le2d::set_error(le2d::ERR_ARG, "server: pair split %d", on);
+  std::lock_guard<std::mutex> lk(h->s->mu);
+  h->s->pair_split = on;
   h->s->dirty = true;
   return le2d::OK;
 }

@@ -130,6 +130,28 @@ __global__ __launch_bounds__(FB, 2) void k_step2_xy(const T *__restrict__ src, T
   march_pair<T, FB>(src, dst, nx, ny, pitch, plane, strip, j0, j1, cx, cy, sw, sf);
 }
 
+// Two fused steps, the stages on two wavefronts (march_pair_split): 128
+// lanes per workgroup, each wavefront one 64-lane window of one stage.
+#ifndef LE2D_SPLIT_WAVES
+#define LE2D_SPLIT_WAVES 3
+#endif
+#ifndef LE2D_SPLIT_HOLD64
+#define LE2D_SPLIT_HOLD64 0
+#endif
+template <typename T, bool HOLD>
+__global__ __launch_bounds__(128, LE2D_SPLIT_WAVES) void k_step2_split(
+    const T *__restrict__ src, T *__restrict__ dst, int nx, int ny, long pitch, long plane,
+    int seg, int nstrips, SweepCoef<T> cx, SweepCoef<T> cy) {
+  __shared__ W4<T> sw[128];
+  __shared__ W4<T> sf[128];
+  __shared__ T ring[2 * 5 * 64];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int strip = tile % nstrips, segi = tile / nstrips;
+  const int j0 = segi * seg;
+  const int j1 = min(j0 + seg, ny);
+  march_pair_split<T, HOLD>(src, dst, nx, ny, pitch, plane, strip, j0, j1, cx, cy, sw, sf, ring);
+}
+
 // ---------------------------------------------------------------------------
 // Launchers.
 
@@ -173,9 +195,16 @@ static int launch_step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T
   const long nsegs = (g->ny + seg - 1) / seg;
   const long nblocks = nsegs * nstrips;
   if (nblocks > 0x7fffffffL) return set_error(ERR_ARG, "step2_xy: grid too large");
-  hipLaunchKernelGGL((k_step2_xy<T, FB>), dim3((unsigned)nblocks), dim3(FB), 0, g->stream,
-                     static_cast<const T *>(g->buf[g->cur]), static_cast<T *>(g->buf[g->cur ^ 1]),
-                     g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx, cy);
+  const T *src = static_cast<const T *>(g->buf[g->cur]);
+  T *dst = static_cast<T *>(g->buf[g->cur ^ 1]);
+  if (FB == 64 && g->tiles.pair_split)
+    hipLaunchKernelGGL((k_step2_split<T, sizeof(T) == 4 || LE2D_SPLIT_HOLD64>),
+                       dim3((unsigned)nblocks), dim3(128), 0,
+                       g->stream, src, dst, g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx,
+                       cy);
+  else
+    hipLaunchKernelGGL((k_step2_xy<T, FB>), dim3((unsigned)nblocks), dim3(FB), 0, g->stream, src,
+                       dst, g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx, cy);
   return OK;
 }
 

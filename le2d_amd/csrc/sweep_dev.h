@@ -486,21 +486,24 @@ __device__ __forceinline__ void march_tile_alt(const T *__restrict__ src, T *__r
 #ifndef LE2D_XS_NOBRANCH
 #define LE2D_XS_NOBRANCH 0
 #endif
+#ifndef LE2D_XS_NOBRANCH_F32
+#define LE2D_XS_NOBRANCH_F32 1
+#endif
 template <typename T, int FB>
 __device__ __forceinline__ Node<T> window_xsweep_cl(Node<T> u, W4<T> *sw, W4<T> *sf, int t,
                                                     const int4 &nb, const SweepCoef<T> &c) {
   const W4<T> w = invariants(u.vx, u.sxx, u.vy, u.sxy, c);
   sw[t] = w;
   window_sync<FB>();
-#if LE2D_XS_NOBRANCH
-  sf[t] = faces(sw[nb.x], sw[nb.y], sw[nb.z], sw[nb.w], c);
-  window_sync<FB>();
-  apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, sf[t], sf[min(t + 1, FB - 1)], c);
-#else
-  if (t >= 2 && t <= FB - 2) sf[t] = faces(sw[nb.x], sw[nb.y], sw[nb.z], sw[nb.w], c);
-  window_sync<FB>();
-  if (t >= 2 && t <= FB - 3) apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, sf[t], sf[t + 1], c);
-#endif
+  if constexpr (LE2D_XS_NOBRANCH || (LE2D_XS_NOBRANCH_F32 && sizeof(T) == 4)) {
+    sf[t] = faces(sw[nb.x], sw[nb.y], sw[nb.z], sw[nb.w], c);
+    window_sync<FB>();
+    apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, sf[t], sf[min(t + 1, FB - 1)], c);
+  } else {
+    if (t >= 2 && t <= FB - 2) sf[t] = faces(sw[nb.x], sw[nb.y], sw[nb.z], sw[nb.w], c);
+    window_sync<FB>();
+    if (t >= 2 && t <= FB - 3) apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, sf[t], sf[t + 1], c);
+  }
   return u;
 }
 
@@ -625,6 +628,175 @@ __device__ __forceinline__ void march_pair(const T *__restrict__ src, T *__restr
 #undef XS2
 #undef XS
 #undef ROW
+}
+
+// ---------------------------------------------------------------------------
+// Two full steps per pass with the two stages on two wavefronts.
+//
+// The window, segment, edge rules and operations of march_pair with a
+// 64-lane window, but stage 1 (x-sweep of the raw rows, y-march to time
+// n+1) runs on wavefront 0 and stage 2 (x-sweep of the time-(n+1) rows,
+// y-march to time n+2) on wavefront 1 of a 128-lane workgroup.  Each time-
+// (n+1) row passes from the first to the second through a two-slot LDS ring,
+// one workgroup barrier per row: wavefront 0 writes row k into slot k & 1 and
+// arrives at barrier k; wavefront 1 reads slot k & 1 after barrier k, before
+// it arrives at barrier k+1, so a slot is rewritten only after its row was
+// read.  Both wavefronts pass exactly j1 - j0 + 4 barriers (one per stage-1
+// row j0-2 .. j1+1).  Each wavefront holds one stage's march state, about
+// half of march_pair's registers, so more of them are resident per SIMD; each
+// x-sweep's LDS exchange stays within its wavefront (wavefront-scope
+// ordering) in its own half of sw / sf.  The result is march_pair's, bit for
+// bit.  sw and sf hold 128 entries each, ring 2 x 5 x 64 values.
+__device__ __forceinline__ void split_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <typename T>
+__device__ __forceinline__ void ring_put(T *slot, int t, const Node<T> &u) {
+  slot[t] = u.vx;
+  slot[64 + t] = u.vy;
+  slot[128 + t] = u.sxx;
+  slot[192 + t] = u.sxy;
+  slot[256 + t] = u.syy;
+}
+
+template <typename T>
+__device__ __forceinline__ Node<T> ring_get(const T *slot, int t) {
+  Node<T> u;
+  u.vx = slot[t];
+  u.vy = slot[64 + t];
+  u.sxx = slot[128 + t];
+  u.sxy = slot[192 + t];
+  u.syy = slot[256 + t];
+  return u;
+}
+
+template <typename T, bool HOLD>
+__device__ __forceinline__ void march_pair_split(const T *__restrict__ src, T *__restrict__ dst,
+                                                 int nx, int ny, long pitch, long plane,
+                                                 int strip, int j0, int j1,
+                                                 const SweepCoef<T> &cx, const SweepCoef<T> &cy,
+                                                 W4<T> *sw, W4<T> *sf, T *ring) {
+  constexpr int FB = 64;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int t = (int)(threadIdx.x & 63);
+  const int x0 = strip * (FB - 8);
+  const int x = x0 - 4 + t;
+  const int tlo = max(0, 4 - x0), thi = min(FB - 1, nx + 3 - x0);
+  const int4 nb = make_int4(min(max(t - 2, tlo), thi), min(max(t - 1, tlo), thi),
+                            min(max(t, tlo), thi), min(max(t + 1, tlo), thi));
+  const int rows = j1 - j0 + 4;  // stage-1 rows j0-2 .. j1+1, one barrier each
+  sw += FB * wave;
+  sf += FB * wave;
+#define XS(n) window_xsweep_cl<T, FB>((n), sw, sf, t, nb, cx)
+  if (wave == 0) {
+    // Stage 1 (as march_pair's): slots A/B/C hold the x-swept time-n rows
+    // q, q+1, q+2, F0/F1 the faces, wb the invariants of row q-1, N0/N1 the
+    // raw rows in flight; HOLD keeps each slot's y-invariants (WA..).
+    const unsigned boff = (unsigned)min(max(x, 0), nx - 1) * (unsigned)sizeof(T);
+#define ROW(p) load_row_node(src + (long)ypos<EDGE_CLAMP>((p), ny) * pitch, plane, boff)
+    Node<T> A, B, C, N0, N1;
+    W4<T> F0, F1, wb, WA, WB, WC;
+    int q = j0 - 2, k = 0;
+    {
+      const W4<T> w2 = inv_y(XS(ROW(q - 2)), cy);
+      wb = inv_y(XS(ROW(q - 1)), cy);
+      A = XS(ROW(q));
+      B = XS(ROW(q + 1));
+      C = XS(ROW(q + 2));
+      N0 = ROW(q + 3);
+      WA = inv_y(A, cy);
+      WB = inv_y(B, cy);
+      if constexpr (HOLD) WC = inv_y(C, cy);
+      F0 = faces(w2, wb, WA, WB, cy);  // face q-1 | q
+    }
+#define P1(P0, P1, P2, W0, W1, W2, FI, FO, NC, NN)                           \
+  {                                                                           \
+    NN = ROW(q + 4);                                                          \
+    const W4<T> w0 = HOLD ? W0 : inv_y(P0, cy);                               \
+    FO = faces(wb, w0, HOLD ? W1 : inv_y(P1, cy), HOLD ? W2 : inv_y(P2, cy), cy); \
+    apply(P0.vy, P0.vx, P0.syy, P0.sxx, P0.sxy, w0, FI, FO, cy);              \
+    wb.a = w0.a;                                                              \
+    wb.c = w0.c;                                                              \
+    ring_put(ring + (k & 1) * 5 * FB, t, P0);                                 \
+    split_sync();                                                             \
+    if (++k >= rows) break;                                                   \
+    P0 = XS(NC);                                                              \
+    if constexpr (HOLD) W0 = inv_y(P0, cy);                                   \
+    ++q;                                                                      \
+  }
+    for (;;) {
+      P1(A, B, C, WA, WB, WC, F0, F1, N0, N1)
+      P1(B, C, A, WB, WC, WA, F1, F0, N1, N0)
+      P1(C, A, B, WC, WA, WB, F0, F1, N0, N1)
+      P1(A, B, C, WA, WB, WC, F1, F0, N1, N0)
+      P1(B, C, A, WB, WC, WA, F0, F1, N0, N1)
+      P1(C, A, B, WC, WA, WB, F1, F0, N1, N0)
+    }
+#undef P1
+#undef ROW
+  } else {
+    // Stage 2: slots A2/B2/C2 hold the x-swept time-(n+1) rows r, r+1, r+2,
+    // G0/G1 the faces, vb the invariants of row r-1.  Positions above row 0
+    // or below row ny-1 take that edge row's value (march_pair's clamps).
+    const unsigned woff = (unsigned)max(x, 0) * (unsigned)sizeof(T);
+    const bool writer = t >= 4 && t <= FB - 5 && x < nx;
+    Node<T> A2, B2, C2, Xm2, Xm1;
+    W4<T> G0, G1, vb, VA, VB, VC;
+    int k = 0;
+#define GET() XS(ring_get(ring + ((k++) & 1) * 5 * FB, t))
+    split_sync();
+    Xm2 = GET();  // row j0-2
+    split_sync();
+    Xm1 = GET();  // row j0-1
+    split_sync();
+    A2 = GET();   // row j0
+    split_sync();
+    B2 = GET();   // row j0+1
+    if (j0 + 1 >= ny) B2 = A2;
+    if (j0 - 1 < 0) Xm1 = A2;
+    if (j0 - 2 < 0) Xm2 = Xm1;
+    vb = inv_y(Xm1, cy);
+    VA = inv_y(A2, cy);
+    VB = inv_y(B2, cy);
+    G0 = faces(inv_y(Xm2, cy), vb, VA, VB, cy);  // face j0-1 | j0
+    int r = j0, q = j0 + 2;
+#define P2(Q0, Q1, Q2, V0, V1, V2, GI, GO)                                    \
+  {                                                                           \
+    split_sync();                                                             \
+    {                                                                         \
+      const Node<T> xn = GET();                                               \
+      if (q >= ny) {                                                          \
+        Q2 = Q1;                                                              \
+        if constexpr (HOLD) V2 = V1;                                          \
+      } else {                                                                \
+        Q2 = xn;                                                              \
+        if constexpr (HOLD) V2 = inv_y(Q2, cy);                               \
+      }                                                                       \
+    }                                                                         \
+    ++q;                                                                      \
+    const W4<T> v0 = HOLD ? V0 : inv_y(Q0, cy);                               \
+    GO = faces(vb, v0, HOLD ? V1 : inv_y(Q1, cy), HOLD ? V2 : inv_y(Q2, cy), cy); \
+    apply(Q0.vy, Q0.vx, Q0.syy, Q0.sxx, Q0.sxy, v0, GI, GO, cy);              \
+    if (writer) store_row_node(dst + (long)r * pitch, plane, woff, Q0);      \
+    if (++r >= j1) break;                                                     \
+    vb.a = v0.a;                                                              \
+    vb.c = v0.c;                                                              \
+  }
+    for (;;) {
+      P2(A2, B2, C2, VA, VB, VC, G0, G1)
+      P2(B2, C2, A2, VB, VC, VA, G1, G0)
+      P2(C2, A2, B2, VC, VA, VB, G0, G1)
+      P2(A2, B2, C2, VA, VB, VC, G1, G0)
+      P2(B2, C2, A2, VB, VC, VA, G0, G1)
+      P2(C2, A2, B2, VC, VA, VB, G1, G0)
+    }
+#undef P2
+#undef GET
+  }
+#undef XS
 }
 
 // ---------------------------------------------------------------------------

@@ -65,6 +65,7 @@ def test_batch_plain_loop_and_shapes_agree():
                          ((256, 1000, "persistent"), False), ((256, 16, "tiles"), False),
                          ((256, 5, "pairs"), False), ((256, 64, "pairs"), False),
                          ((128, 9, "pairs"), False), ((64, 11, "pairs"), False),
+                         ((64, 3, "pairs_split"), False), ((64, 64, "pairs_split"), False),
                          ((128, 32, None), True)]:
         b, states = _make(np.float64)
         with b:
@@ -84,7 +85,9 @@ def test_batch_autotune_keeps_results():
     with b:
         fb, seg, ms = b.autotune(1)  # advances every instance by an even count
         assert fb in (64, 128, 256) and seg in (16, 32, 64, 128) and ms > 0
-        n_tuned = 2 * 3 * 4 * 4 + 2 * 4 * 8  # single-step shapes x 4 steps + two-step x 8
+        # single-step shapes x 4 steps + two-step shapes (three windows and
+        # the split form) x 8
+        n_tuned = 2 * 3 * 4 * 4 + (3 + 1) * 4 * 8
         for k, g in enumerate(states):
             b.load(k, g)
         b.step(3)
@@ -100,9 +103,11 @@ def test_batch_fp32_close_to_fp64():
         refs = _oracle(states, 20)
         for k, r in enumerate(refs):
             got = b.read(k).astype(np.float64)
-            # FP32 storage and arithmetic against the FP64 oracle
+            # FP32 storage and arithmetic against the FP64 oracle: the FP32
+            # tier's bar, relative L2 1e-5 (the reference's own float build
+            # stays within 4.2e-7 on these cases)
             l2 = np.linalg.norm(got - r) / np.linalg.norm(r)
-            assert l2 < 2e-5, (SIZES[k], l2)
+            assert l2 < 1e-5, (SIZES[k], l2)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])

@@ -1,7 +1,8 @@
 """Two steps per pass (le2d_set_step_pairs): bit-identical to single steps for
 every size (one and several strips, grids narrower than a window, one to a
 few rows), segment height (1 row to more than the grid), window width (64,
-128 and 256 lanes) and element type,
+128 and 256 lanes; 64-lane windows with the stages on one or two
+wavefronts) and element type,
 odd step counts included (the last step takes the single-step pass), and
 to the oracle in FP64."""
 import numpy as np
@@ -19,14 +20,16 @@ SEGS = (0, 1, 3, 7, 32, 1000)
 ARGS = (0.2, (1.0, 0.9), (2.0, 1.0, 1.5))
 
 
-def _run(g0, dtype, seg, steps, lanes=256):
+def _run(g0, dtype, seg, steps, lanes=256, split=0):
     ny, nx = g0.shape[:2]
     with le2d_amd.Task(nx, ny, *ARGS, dtype=dtype) as t:
         t.load(g0)
         t.set_step_pairs(seg)
         t.set_step_pair_window(lanes)
+        t.set_step_pair_split(split)
         assert t.get_step_pairs() == seg
         assert t.get_step_pair_window() == lanes
+        assert t.get_step_pair_split() == split
         t.step_n(steps)
         return t.grid().copy()
 
@@ -37,15 +40,16 @@ def test_pairs_equal_single_steps(dtype, nx, ny):
     rng = np.random.default_rng(1000 * nx + ny)
     g0 = rng.standard_normal((ny, nx, 5)).astype(dtype)
     single = _run(g0, dtype, 0, 5)
-    for lanes in (256, 128, 64):
+    for lanes, split in ((256, 0), (128, 0), (64, 0), (64, 1)):
         for seg in SEGS[1:]:
-            assert gd.same_bits(_run(g0, dtype, seg, 5, lanes), single), (lanes, seg)
+            assert gd.same_bits(_run(g0, dtype, seg, 5, lanes, split), single), (lanes, split, seg)
 
 
-def test_pairs_match_oracle():
+@pytest.mark.parametrize("lanes,split", [(256, 0), (64, 0), (64, 1)])
+def test_pairs_match_oracle(lanes, split):
     rng = np.random.default_rng(5)
     g0 = rng.standard_normal((131, 517, 5))
-    got = _run(g0, np.float64, 16, 6)
+    got = _run(g0, np.float64, 16, 6, lanes, split)
     ref = g0.copy()
     Oracle(np.float64).run(ref, *ARGS, 6)
     assert gd.same_bits(got, ref)

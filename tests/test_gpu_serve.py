@@ -106,10 +106,12 @@ def test_budget_limits_admission_and_rejects_oversize():
 def test_two_step_launches_change_nothing():
     # odd chunk limit and odd step count: two-step launches plus single ones
     res = []
-    for pairs, lanes in ((0, 0), (64, 0), (5, 64), (16, 128), (7, 256)):
+    for pairs, lanes, split in ((0, 0, 0), (64, 0, 0), (5, 64, 0), (16, 128, 0), (7, 256, 0),
+                                (5, 64, 1), (64, 64, 1)):
         with le2d_amd.Server(1 * GB, 16) as s:
             s.set_pairs(pairs)
             s.set_pair_window(lanes)
+            s.set_pair_split(split)
             s.set_shape(256, 16, 7)
             ids = [s.submit(**REQ, dtype=dt, keep=True) for dt in (np.float32, np.float64)]
             s.drain(60)
@@ -161,6 +163,7 @@ def test_settings_changed_while_running():
         def toggle():
             for k in range(40):
                 s.set_pair_window((64, 128, 256, 0)[k % 4])
+                s.set_pair_split(k % 3 == 1)
                 s.set_pairs((8, 64, 3, 16)[k % 4])
                 s.set_shape((64, 128, 256)[k % 3], 16, 4)
                 time.sleep(0.0005)
