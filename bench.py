@@ -97,27 +97,84 @@ def sustained(t, n, steps, el, seconds):
     return {"steps": k, "seconds": round(dt, 3), "Gnode_per_s": round(n * n * k / dt / 1e9, 3)}
 
 
-def dense_report(le2d_amd, n, dtype, steps=200, seed=5):
-    """The same grid with all five fields random (uniform in [-1, 1]): the
-    kernels have no data-dependent path, so the rate matches the disk
-    workload's."""
+def gpu_sensors():
+    """Shader clock (MHz) and package power (W) of the first GPU from sysfs,
+    where the driver exposes them; {} otherwise."""
+    import glob
+    for d in sorted(glob.glob("/sys/class/drm/card*/device")):
+        out = {}
+        try:
+            cur = [ln for ln in open(d + "/pp_dpm_sclk").read().splitlines()
+                   if ln.strip().endswith("*")]
+            if cur:
+                out["sclk_MHz"] = float(cur[0].split()[1].lower().rstrip("mhz"))
+        except (OSError, ValueError, IndexError):
+            pass
+        for h in glob.glob(d + "/hwmon/hwmon*"):
+            for f in ("power1_input", "power1_average"):
+                try:
+                    out["power_W"] = int(open(os.path.join(h, f)).read()) / 1e6
+                    break
+                except (OSError, ValueError):
+                    pass
+        if out:
+            return out
+    return {}
+
+
+def sampled(fn, period=0.02):
+    """Run fn() while a thread samples gpu_sensors(); (result, medians)."""
+    import threading
+    samples, stop = [], [False]
+
+    def loop():
+        while not stop[0]:
+            samples.append(gpu_sensors())
+            time.sleep(period)
+    th = threading.Thread(target=loop, daemon=True)
+    th.start()
+    try:
+        res = fn()
+    finally:
+        stop[0] = True
+        th.join()
+    med = {}
+    for k in ("sclk_MHz", "power_W"):
+        v = [x[k] for x in samples if k in x]
+        if v:
+            med[k] = round(float(np.median(v)), 1)
+    return res, med
+
+
+def dense_report(le2d_amd, n, dtype, shape, passes=150, seed=5):
+    """The same grid, the same launch shape, on the bench's disk field and
+    with all five fields random (uniform in [-1, 1]): the kernels have no
+    data-dependent path (the same instructions and cycles per launch), so
+    any difference in rate comes from the clock the device holds; the shader
+    clock and the power are sampled from sysfs during each timed run."""
+    out = {"workload": f"{n}x{n}, {passes} two-step passes at the headline's launch shape "
+                       f"(pair_seg, lanes, split = {shape}), on the disk field and on all five "
+                       f"fields uniform in [-1, 1] (seed {seed})"}
     rng = np.random.default_rng(seed)
-    with le2d_amd.Task(n, n, 0.25, (1.0, 1.0), (2.0, 1.0, 1.0), dtype=dtype) as t:
-        v = t.view()
-        for j in range(0, n, 512):  # fill in slabs (bounded host temporaries)
-            v[j:j + 512] = rng.uniform(-1.0, 1.0, size=v[j:j + 512].shape)
-        t.lib.le_mark_host_dirty(t.ref)
-        if os.environ.get("LE2D_TUNE") != "0" and t.get_step_pairs() > 0:
-            t.tune_step_pairs()
-        t.step_n(10)
-        t.wait()
-        t0 = time.perf_counter()
-        t.step_n(steps)
-        t.wait()
-        dt = time.perf_counter() - t0
-    return {"workload": f"{n}x{n}, all five fields uniform in [-1, 1] (seed {seed}), "
-                        f"{steps} steps through le_step_n",
-            "Gnode_per_s": round(n * n * steps / dt / 1e9, 3), "seconds": round(dt, 3)}
+    for kind in ("disk", "dense"):
+        with le2d_amd.Task(n, n, 0.25, (1.0, 1.0), (2.0, 1.0, 1.0), dtype=dtype) as t:
+            if kind == "disk":
+                t.set_ball(disk_centre(n), 64.0, 1.0)
+            else:
+                v = t.view()
+                for j in range(0, n, 512):  # fill in slabs (bounded host temporaries)
+                    v[j:j + 512] = rng.uniform(-1.0, 1.0, size=v[j:j + 512].shape)
+                t.lib.le_mark_host_dirty(t.ref)
+            t.set_step_pairs(shape[0])
+            t.set_step_pair_window(shape[1])
+            t.set_step_pair_split(shape[2])
+            t.step_n(10)
+            t.wait()
+            ms, sens = sampled(lambda: t.time_pairs(passes))
+        out[kind] = dict(Gnode_per_s=round(2 * n * n / ms / 1e6, 3), ms_per_pass=round(ms, 4),
+                         **sens)
+    out["dense_vs_disk"] = round(out["dense"]["Gnode_per_s"] / out["disk"]["Gnode_per_s"], 3)
+    return out
 
 
 def repeat_stats(n, steps, els):
@@ -519,7 +576,7 @@ def serial_sample(reqs, every):
 
 
 def serve_report(le2d_amd, window_s=60.0, budget_gb=200.0, slots=1024, chunk=16,
-                 serial_every=20, plain_rates=None, split=0, serial=True):
+                 serial_every=20, plain_rates=None, split=0, serial=True, batch_rates=None):
     """Continuous batching of a Poisson request stream against the serial
     plain path.  The batched server receives the requests in real time.  The
     serial path runs a stratified sample (one request in `serial_every`,
@@ -527,7 +584,9 @@ def serve_report(le2d_amd, window_s=60.0, budget_gb=200.0, slots=1024, chunk=16,
     single-grid kernels; each sampled request's service rate stands for its
     stratum, and the serial queue is then replayed on the same arrivals.
     `plain_rates` (FP32, FP64 Gnode-updates/s of one large grid) gives the
-    work-weighted ceiling of the mix."""
+    work-weighted ceiling of the mix; `batch_rates` (the mixed-size batch's
+    FP32 and FP64 rates) the same ceiling at the rates of grids of the
+    requests' sizes."""
     reqs = serve_requests(window_s)
     work = sum(q["nx"] * q["ny"] * q["steps"] for q in reqs)
     w32 = sum(q["nx"] * q["ny"] * q["steps"] for q in reqs if q["dtype"] == np.float32)
@@ -622,6 +681,14 @@ def serve_report(le2d_amd, window_s=60.0, budget_gb=200.0, slots=1024, chunk=16,
         out["ceiling_basis"] = (f"work-weighted: FP32 work at {r32:.1f} and FP64 work at "
                                 f"{r64:.1f} Gnode-updates/s, the plain path on one 8192^2 grid")
         out["served_vs_ceiling"] = round(served / ceiling, 3)
+    if batch_rates:
+        r32, r64 = batch_rates
+        ceiling = work / 1e9 / (w32 / 1e9 / r32 + (work - w32) / 1e9 / r64)
+        out["blended_batch_ceiling_Gnode_per_s"] = round(ceiling, 3)
+        out["batch_ceiling_basis"] = (f"work-weighted: FP32 work at {r32:.1f} and FP64 work at "
+                                      f"{r64:.1f} Gnode-updates/s, the 256-instance mixed-size "
+                                      f"batch (sides 128..2048)")
+        out["served_vs_batch_ceiling"] = round(served / ceiling, 3)
     return out
 
 
@@ -710,18 +777,20 @@ def main():
         other["repeats"] = repeat_stats(n, args.steps,
                                         els2 + time_runs(t2, args.steps, args.repeats - 1))
         other["sustained"] = sustained(t2, n, args.steps, el2, args.sustain)
+        shape2 = (t2.get_step_pairs(), t2.get_step_pair_window(), t2.get_step_pair_split())
         t2.close()
         if not args.no_dense:
             log(f"{n}x{n} {o_name} dense field")
-            other["dense_field"] = dense_report(le2d_amd, n, o_dt)
+            other["dense_field"] = dense_report(le2d_amd, n, o_dt, shape2)
     els += time_runs(t, args.steps, args.repeats - 1)
     log(f"{n}x{n} {args.dtype}: sustained {args.sustain:g} s")
     sus = sustained(t, n, args.steps, el, args.sustain)
+    shape = (t.get_step_pairs(), t.get_step_pair_window(), t.get_step_pair_split())
     t.close()
     dense = None
     if not args.no_dense:
         log(f"{n}x{n} {args.dtype} dense field")
-        dense = dense_report(le2d_amd, n, main_dt)
+        dense = dense_report(le2d_amd, n, main_dt, shape)
     copy_bytes = n * n * 5 * es
     copy_gbs = 2 * copy_bytes / le2d_amd.copy_probe(copy_bytes, 10) / 1e6
 
@@ -745,7 +814,10 @@ def main():
             rates = (r32, r64)
         serving = serve_report(le2d_amd, window_s=args.serve_window, chunk=args.serve_chunk,
                                serial_every=args.serial_every, plain_rates=rates,
-                               split=args.serve_split, serial=not args.no_serial)
+                               split=args.serve_split, serial=not args.no_serial,
+                               batch_rates=(batched["fp32"]["batched_Gnode_per_s"],
+                                            batched["fp64"]["batched_Gnode_per_s"])
+                               if batched else None)
     cpu = None
     if not args.no_cpu:
         log("CPU baseline")

@@ -159,3 +159,35 @@ def test_batch_long_trajectory_matches_oracle():
         b.step(31)
         for k, r in enumerate(_oracle(inits, 31)):
             assert gd.same_bits(b.read(k), r), SIZES[k]
+
+
+def test_fp32_batch_error_is_the_reference_float_builds():
+    """Three instances of the bench's mixed batch (indices 19, 40, 50; 554
+    steps from their disks, as in the bench): the FP32 batch departs from
+    FP64 by more than 1e-5 relative L2 on them, and so does the reference's
+    own float build (the oracle compiled with real = float), by about as
+    much.  The FP32 tier's error is the scheme's sensitivity in single
+    precision, not the GPU's arithmetic."""
+    import bench
+    insts = [bench.mixed_instances(256)[k] for k in (19, 40, 50)]
+    OF = Oracle(np.float32)
+    b = le2d_amd.Batch(np.float32)
+    for p in insts:
+        b.add(p["nx"], p["ny"], p["dt"], (1.0, 1.0), (p["c1"], p["c2"], p["rho"]))
+    b.commit()
+    with b:
+        for i, p in enumerate(insts):
+            b.set_ball(i, (p["cx"], p["cy"]), p["r"], 1.0)
+        b.step(554)
+        for i, p in enumerate(insts):
+            mat = (p["c1"], p["c2"], p["rho"])
+            ref = np.zeros((p["ny"], p["nx"], 5))
+            OD.set_ball(ref, (1.0, 1.0), (p["cx"], p["cy"]), p["r"], 1.0)
+            flt = np.zeros((p["ny"], p["nx"], 5), np.float32)  # the disk in float, as the batch
+            OF.set_ball(flt, (1.0, 1.0), (p["cx"], p["cy"]), p["r"], 1.0)
+            OD.run(ref, p["dt"], (1.0, 1.0), mat, 554)
+            OF.run(flt, p["dt"], (1.0, 1.0), mat, 554)
+            e_ref = np.linalg.norm(flt.astype(np.float64) - ref) / np.linalg.norm(ref)
+            e_gpu = np.linalg.norm(b.read(i).astype(np.float64) - ref) / np.linalg.norm(ref)
+            assert e_ref > 1e-5 and e_gpu > 1e-5, (i, e_ref, e_gpu)
+            assert e_gpu < 1.5 * e_ref, (i, e_ref, e_gpu)
