@@ -138,18 +138,22 @@ __global__ __launch_bounds__(FB, 2) void k_step2_xy(const T *__restrict__ src, T
 #ifndef LE2D_SPLIT_HOLD64
 #define LE2D_SPLIT_HOLD64 0
 #endif
+#ifndef LE2D_SPLIT_RB
+#define LE2D_SPLIT_RB 1
+#endif
 template <typename T, bool HOLD>
 __global__ __launch_bounds__(128, LE2D_SPLIT_WAVES) void k_step2_split(
     const T *__restrict__ src, T *__restrict__ dst, int nx, int ny, long pitch, long plane,
     int seg, int nstrips, SweepCoef<T> cx, SweepCoef<T> cy) {
   __shared__ W4<T> sw[128];
   __shared__ W4<T> sf[128];
-  __shared__ T ring[2 * 5 * 64];
+  __shared__ T ring[2 * LE2D_SPLIT_RB * 5 * 64];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int strip = tile % nstrips, segi = tile / nstrips;
   const int j0 = segi * seg;
   const int j1 = min(j0 + seg, ny);
-  march_pair_split<T, HOLD>(src, dst, nx, ny, pitch, plane, strip, j0, j1, cx, cy, sw, sf, ring);
+  march_pair_split<T, HOLD, LE2D_SPLIT_RB>(src, dst, nx, ny, pitch, plane, strip, j0, j1, cx, cy,
+                                           sw, sf, ring);
 }
 
 // ---------------------------------------------------------------------------
@@ -276,9 +280,9 @@ int sweep_x(DevGrid *g, const SweepCoef<T> &c) {
   constexpr int BX = 256;
   // gridDim.y is at most 65535
   const int rows = max(max(1, g->tiles.x_rows), (g->ny + 65534) / 65535);
-  dim3 grid((g->nx + BX - 1) / BX, (g->ny + rows - 1) / rows);
   const T *src = static_cast<const T *>(g->buf[g->cur]);
   T *dst = static_cast<T *>(g->buf[g->cur ^ 1]);
+  dim3 grid((g->nx + BX - 1) / BX, (g->ny + rows - 1) / rows);
   hipLaunchKernelGGL((k_sweep_x<T, BX>), grid, dim3(BX), 0, g->stream, src, dst, g->nx, g->ny,
                      g->pitch, g->plane, rows, c);
   hipError_t e = hipGetLastError();

@@ -637,16 +637,19 @@ __device__ __forceinline__ void march_pair(const T *__restrict__ src, T *__restr
 // 64-lane window, but stage 1 (x-sweep of the raw rows, y-march to time
 // n+1) runs on wavefront 0 and stage 2 (x-sweep of the time-(n+1) rows,
 // y-march to time n+2) on wavefront 1 of a 128-lane workgroup.  Each time-
-// (n+1) row passes from the first to the second through a two-slot LDS ring,
-// one workgroup barrier per row: wavefront 0 writes row k into slot k & 1 and
-// arrives at barrier k; wavefront 1 reads slot k & 1 after barrier k, before
-// it arrives at barrier k+1, so a slot is rewritten only after its row was
-// read.  Both wavefronts pass exactly j1 - j0 + 4 barriers (one per stage-1
-// row j0-2 .. j1+1).  Each wavefront holds one stage's march state, about
+// (n+1) row passes from the first to the second through an LDS ring of 2*RB
+// slots, one workgroup barrier per RB rows: wavefront 0 writes row k into slot
+// k % (2*RB) and arrives at barrier k / RB after the last row of that group
+// (or the last row of all); wavefront 1 passes barrier b before it reads rows
+// b*RB .. b*RB+RB-1, and passes barrier b+1 only after reading them, while
+// wavefront 0 fills the other half of the ring; so a slot is rewritten only
+// after its row was read.  Both wavefronts pass exactly ceil((j1 - j0 + 4) /
+// RB) barriers (stage-1 rows j0-2 .. j1+1).  Each wavefront holds one stage's
+// march state, about
 // half of march_pair's registers, so more of them are resident per SIMD; each
 // x-sweep's LDS exchange stays within its wavefront (wavefront-scope
 // ordering) in its own half of sw / sf.  The result is march_pair's, bit for
-// bit.  sw and sf hold 128 entries each, ring 2 x 5 x 64 values.
+// bit.  sw and sf hold 128 entries each, ring 2*RB x 5 x 64 values.
 __device__ __forceinline__ void split_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_s_barrier();
@@ -673,7 +676,7 @@ __device__ __forceinline__ Node<T> ring_get(const T *slot, int t) {
   return u;
 }
 
-template <typename T, bool HOLD>
+template <typename T, bool HOLD, int RB = 1>
 __device__ __forceinline__ void march_pair_split(const T *__restrict__ src, T *__restrict__ dst,
                                                  int nx, int ny, long pitch, long plane,
                                                  int strip, int j0, int j1,
@@ -687,7 +690,8 @@ __device__ __forceinline__ void march_pair_split(const T *__restrict__ src, T *_
   const int tlo = max(0, 4 - x0), thi = min(FB - 1, nx + 3 - x0);
   const int4 nb = make_int4(min(max(t - 2, tlo), thi), min(max(t - 1, tlo), thi),
                             min(max(t, tlo), thi), min(max(t + 1, tlo), thi));
-  const int rows = j1 - j0 + 4;  // stage-1 rows j0-2 .. j1+1, one barrier each
+  const int rows = j1 - j0 + 4;  // stage-1 rows j0-2 .. j1+1
+  constexpr int RS = 2 * RB;     // ring slots
   sw += FB * wave;
   sf += FB * wave;
 #define XS(n) window_xsweep_cl<T, FB>((n), sw, sf, t, nb, cx)
@@ -720,8 +724,8 @@ __device__ __forceinline__ void march_pair_split(const T *__restrict__ src, T *_
     apply(P0.vy, P0.vx, P0.syy, P0.sxx, P0.sxy, w0, FI, FO, cy);              \
     wb.a = w0.a;                                                              \
     wb.c = w0.c;                                                              \
-    ring_put(ring + (k & 1) * 5 * FB, t, P0);                                 \
-    split_sync();                                                             \
+    ring_put(ring + (k % RS) * 5 * FB, t, P0);                                \
+    if (k % RB == RB - 1 || k == rows - 1) split_sync();                      \
     if (++k >= rows) break;                                                   \
     P0 = XS(NC);                                                              \
     if constexpr (HOLD) W0 = inv_y(P0, cy);                                   \
@@ -746,14 +750,11 @@ __device__ __forceinline__ void march_pair_split(const T *__restrict__ src, T *_
     Node<T> A2, B2, C2, Xm2, Xm1;
     W4<T> G0, G1, vb, VA, VB, VC;
     int k = 0;
-#define GET() XS(ring_get(ring + ((k++) & 1) * 5 * FB, t))
-    split_sync();
+    // barrier before each group of RB rows, then row k from its slot
+#define GET() (k % RB == 0 ? split_sync() : (void)0, XS(ring_get(ring + ((k++) % RS) * 5 * FB, t)))
     Xm2 = GET();  // row j0-2
-    split_sync();
     Xm1 = GET();  // row j0-1
-    split_sync();
     A2 = GET();   // row j0
-    split_sync();
     B2 = GET();   // row j0+1
     if (j0 + 1 >= ny) B2 = A2;
     if (j0 - 1 < 0) Xm1 = A2;
@@ -765,7 +766,6 @@ __device__ __forceinline__ void march_pair_split(const T *__restrict__ src, T *_
     int r = j0, q = j0 + 2;
 #define P2(Q0, Q1, Q2, V0, V1, V2, GI, GO)                                    \
   {                                                                           \
-    split_sync();                                                             \
     {                                                                         \
       const Node<T> xn = GET();                                               \
       if (q >= ny) {                                                          \
