@@ -508,7 +508,10 @@ def tier_report(le2d_amd, n=16384, steps=2000, seed=7, every=250):
         out[name] = {"Gnode_per_s": round(n * n * steps / el / 1e9, 3), "seconds": round(el, 3),
                      "energy_initial": e0, "energy_final": e1,
                      "energy_rel_drift": (e1 - e0) / e0,
-                     f"energy_every_{every}_steps": curve}
+                     f"energy_every_{every}_steps": curve,
+                     "tiles": {"pair_seg": t.get_step_pairs(),
+                               "pair_lanes": t.get_step_pair_window(),
+                               "pair_split": t.get_step_pair_split()}}
         tasks[name] = t
     norms = le2d_amd.diff_norms(tasks["fp32"].device_view(), tasks["fp64"].device_view())
     out["fp32"].update({"vs_fp64_" + k: v for k, v in rel_errors(norms).items()})

@@ -42,6 +42,7 @@ struct TaskState {
   int mode = LE2D_SYNC_WAIT;
   hipStream_t stream = nullptr;
   bool pairs_fixed = false;  // segment height set by the caller or tuned
+  bool mat_split_tuned = false;  // material plane: one or two wavefronts chosen
   // A step accepted by le_step but not yet run, so that it and the next
   // le_step share one two-step pass; run before anything else touches the
   // device state (flush_pending), with the coefficients it was called with.
@@ -251,6 +252,38 @@ int tune_pairs(le_task *t, TaskState *s) {
   return LE2D_OK;
 }
 
+// Material planes keep their default window and segment height (other
+// shapes would rebuild the tile tables on the host), but with 64-lane windows
+// the two forms of the pass share the tables: time each over three passes,
+// out of place (each pass reads the current state and writes the idle
+// buffer; the swap is undone), and keep the faster.
+int tune_mat_split(TaskState *s) {
+  hipEvent_t ev[2];
+  for (auto &e : ev) (void)hipEventCreate(&e);
+  int rc = LE2D_OK, pick = 0;
+  float best = 1e30f;
+  for (int split : {0, 1}) {
+    s->g.tiles.pair_split = split;
+    for (int k = 0; k <= 3 && !rc; ++k) {
+      if (k == 1) (void)hipEventRecord(ev[0], s->stream);
+      rc = le2d::step2_xy_mat<real>(&s->g, &s->mat);
+      if (!rc) s->g.cur ^= 1;  // the state has not moved
+    }
+    (void)hipEventRecord(ev[1], s->stream);
+    if (rc || hipEventSynchronize(ev[1]) != hipSuccess) break;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+    if (ms < best) {
+      best = ms;
+      pick = split;
+    }
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  s->g.tiles.pair_split = pick;
+  s->mat_split_tuned = true;
+  return rc;
+}
+
 bool tuning_enabled() {
   const char *e = getenv("LE2D_TUNE");
   return !(e && !strcmp(e, "0"));
@@ -304,6 +337,12 @@ int run(le_task *t, int what, int edge, int steps, bool defer = false) {
   if (what == (DO_X | DO_Y) && edge == le2d::EDGE_CLAMP && s->g.tiles.pair_seg > 0 &&
       !s->pairs_fixed && !s->mat.active && steps >= 8 && tuning_enabled()) {
     rc = tune_pairs(t, s);
+    if (rc) return rc;
+  }
+  if (what == (DO_X | DO_Y) && edge == le2d::EDGE_CLAMP && s->g.tiles.pair_seg > 0 &&
+      s->g.tiles.pair_block == 64 && !s->pairs_fixed && s->mat.active && !s->mat_split_tuned &&
+      steps >= 8 && tuning_enabled()) {
+    rc = tune_mat_split(s);
     if (rc) return rc;
   }
   if (what == (DO_X | DO_Y) && edge == le2d::EDGE_CLAMP && s->g.tiles.pair_seg > 0) {
@@ -755,6 +794,7 @@ int le2d_set_material_plane(le_task *task, int count, const double *c1_c2_rho,
     s->mat.active = false;
     return LE2D_OK;
   }
+  s->mat_split_tuned = false;
   return le2d::mat_set(&s->g, &s->mat, count, c1_c2_rho, ids, (double)task->dt,
                        (double)task->h.x, (double)task->h.y);
 }

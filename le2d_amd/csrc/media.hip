@@ -63,6 +63,22 @@ __global__ __launch_bounds__(FB, 2) void k_step2_mat_homog(const T *__restrict__
   march_pair<T, FB>(src, dst, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cx, cy, sw, sf);
 }
 
+// The same for 64-lane windows with the two stages on two wavefronts
+// (march_pair_split).
+template <typename T>
+__global__ __launch_bounds__(128, 3) void k_step2_mat_homog_split(
+    const T *__restrict__ src, T *__restrict__ dst, int nx, int ny, long pitch, long plane,
+    const MatTile *__restrict__ tiles, const SweepCoef<T> *__restrict__ cxt,
+    const SweepCoef<T> *__restrict__ cyt) {
+  __shared__ W4<T> sw[128];
+  __shared__ W4<T> sf[128];
+  __shared__ T ring[2 * 5 * 64];
+  const MatTile tj = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const SweepCoef<T> cx = cxt[tj.mat], cy = cyt[tj.mat];
+  march_pair_split<T, sizeof(T) == 4>(src, dst, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cx,
+                                      cy, sw, sf, ring);
+}
+
 template <typename T, int FB>
 __global__ __launch_bounds__(FB) void k_step2_mat_mixed(const T *__restrict__ src,
                                                        T *__restrict__ dst, int nx, int ny,
@@ -494,7 +510,10 @@ static int launch_mat2(DevGrid *g, MatPlane *m) {
                        dst, g->nx, g->ny, g->pitch, g->plane, m->d_mixed2, m->d_ids, m->idp, cx,
                        cy);
   }
-  if (m->n_homog2)
+  if (m->n_homog2 && FB == 64 && g->tiles.pair_split)
+    hipLaunchKernelGGL((k_step2_mat_homog_split<T>), dim3(m->n_homog2), dim3(128), 0, g->stream,
+                       src, dst, g->nx, g->ny, g->pitch, g->plane, m->d_homog2, cx, cy);
+  else if (m->n_homog2)
     hipLaunchKernelGGL((k_step2_mat_homog<T, FB>), dim3(m->n_homog2), dim3(FB), 0, g->stream, src,
                        dst, g->nx, g->ny, g->pitch, g->plane, m->d_homog2, cx, cy);
   return m->n_mixed2 ? join_aux(g, m) : OK;

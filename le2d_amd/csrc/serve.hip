@@ -737,14 +737,16 @@ int le2d_server_cancel(le2d_server *h, long long id) {
   if (it == s->reqs.end()) return le2d::set_error(le2d::ERR_ARG, "server: no request %lld", id);
   le2d::Req &r = it->second;
   switch (r.state) {
-    case le2d::QUEUED:
-      s->queue.erase(std::find(s->queue.begin(), s->queue.end(), id));
+    case le2d::QUEUED: {
+      auto q = std::find(s->queue.begin(), s->queue.end(), id);
+      if (q != s->queue.end()) s->queue.erase(q);
       r.state = le2d::CANCELLED;
       r.t_done = le2d::now_s();
       ++s->cancelled;
       s->cv_done.notify_all();
       s->cv_work.notify_one();
       return le2d::OK;
+    }
     case le2d::LIVE:
       r.cancel = true;  // the engine drops it and frees its memory after the chunk
       return le2d::OK;

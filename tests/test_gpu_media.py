@@ -194,11 +194,13 @@ def test_heterogeneous_pairs_equal_single_steps(shape, kind):
     g0 = rng.standard_normal((ny, nx, 5))
     dt, h = 0.45 / 3.1, (1.0, 0.9)
     res = []
-    for seg, lanes in ((0, 256), (1, 256), (6, 128), (64, 256), (3, 64), (64, 64)):
+    for seg, lanes, split in ((0, 256, 0), (1, 256, 0), (6, 128, 0), (64, 256, 0), (3, 64, 0),
+                              (64, 64, 0), (3, 64, 1), (64, 64, 1)):
         with _task(g0, dt, h, (2.0, 1.0, 1.0)) as t:
             t.set_materials(tab, ids)
             t.set_step_pairs(seg)
             t.set_step_pair_window(lanes)
+            t.set_step_pair_split(split)
             t.step_n(5)
             res.append(t.grid().copy())
     for r in res[1:]:
@@ -263,20 +265,22 @@ def test_heterogeneous_fp32_within_tier_bound(shape, kind, steps):
     ref = g0.copy()
     OD.run_mat(ref, dt, h, TAB3, ids, steps)
     single = None
-    for seg, lanes, xy_block in ((0, 256, 256), (0, 256, 64), (64, 256, 256), (6, 128, 256),
-                                 (3, 64, 256), (64, 64, 256)):
+    for seg, lanes, xy_block, split in ((0, 256, 256, 0), (0, 256, 64, 0), (64, 256, 256, 0),
+                                        (6, 128, 256, 0), (3, 64, 256, 0), (64, 64, 256, 0),
+                                        (5, 64, 256, 1)):
         with _task(g0, dt, h, (2.0, 1.0, 1.0), dtype=np.float32) as t:
             t.set_tiles(xy_block=xy_block, xy_seg=11)
             t.set_materials(TAB3, ids)
             t.set_step_pairs(seg)
             t.set_step_pair_window(lanes)
+            t.set_step_pair_split(split)
             t.step_n(steps)
             got = t.grid().copy()
-        _fp32_close(got, ref, (seg, lanes, xy_block))
+        _fp32_close(got, ref, (seg, lanes, xy_block, split))
         if single is None:
             single = got
         else:
-            _fp32_close(got, single.astype(np.float64), ("vs single", seg, lanes))
+            _fp32_close(got, single.astype(np.float64), ("vs single", seg, lanes, split))
 
 
 def test_heterogeneous_fp32_trajectory_across_interfaces():
