@@ -80,24 +80,7 @@ __device__ __forceinline__ T superbee(T r) {
 #ifndef LE2D_FP32_NAN2
 #define LE2D_FP32_NAN2 0
 #endif
-// LE2D_DIV_UNSCALED = 1 (measurement only, not a supported build): the FP64
-// division without its operand scaling, exact only while the operands'
-// exponents stay in the range where the scaling is an identity.
-#ifndef LE2D_DIV_UNSCALED
-#define LE2D_DIV_UNSCALED 0
-#endif
-__device__ __forceinline__ double slope_ratio(double n, double d) {
-#if LE2D_DIV_UNSCALED
-  double rc = __builtin_amdgcn_rcp(d);
-  rc = __builtin_fma(rc, __builtin_fma(-d, rc, 1.0), rc);
-  rc = __builtin_fma(rc, __builtin_fma(-d, rc, 1.0), rc);
-  double q = n * rc;
-  q = __builtin_fma(__builtin_fma(-d, q, n), rc, q);
-  return __builtin_amdgcn_div_fixup(q, d, n);
-#else
-  return n / d;
-#endif
-}
+__device__ __forceinline__ double slope_ratio(double n, double d) { return n / d; }
 __device__ __forceinline__ float slope_ratio(float n, float d) {
 #if LE2D_FP32_RATIO == 2
   return n / d;
@@ -480,22 +463,17 @@ __device__ __forceinline__ void march_tile_alt(const T *__restrict__ src, T *__r
 // x-sweep of one window row with clamped neighbour lanes: the face between
 // lanes t-1 and t reads lanes nb = (t-2, t-1, t, t+1), each clamped into the
 // lanes holding grid columns.  Meaningful on lanes 2..FB-3.
-// LE2D_XS_NOBRANCH = 1: every lane forms a face and applies (lanes outside
-// 2..FB-3 produce values no output reads; their LDS indices stay in the
-// window), so the x-sweep has no divergent branch.
-#ifndef LE2D_XS_NOBRANCH
-#define LE2D_XS_NOBRANCH 0
-#endif
-#ifndef LE2D_XS_NOBRANCH_F32
-#define LE2D_XS_NOBRANCH_F32 1
-#endif
+// FP32: every lane forms a face and applies (lanes outside 2..FB-3 produce
+// values no output reads; their LDS indices stay in the window), so the
+// x-sweep has no divergent branch (1.9% faster two-step pass).  FP64 keeps
+// the branches: without them the one-wave pass spills.
 template <typename T, int FB>
 __device__ __forceinline__ Node<T> window_xsweep_cl(Node<T> u, W4<T> *sw, W4<T> *sf, int t,
                                                     const int4 &nb, const SweepCoef<T> &c) {
   const W4<T> w = invariants(u.vx, u.sxx, u.vy, u.sxy, c);
   sw[t] = w;
   window_sync<FB>();
-  if constexpr (LE2D_XS_NOBRANCH || (LE2D_XS_NOBRANCH_F32 && sizeof(T) == 4)) {
+  if constexpr (sizeof(T) == 4) {
     sf[t] = faces(sw[nb.x], sw[nb.y], sw[nb.z], sw[nb.w], c);
     window_sync<FB>();
     apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, sf[t], sf[min(t + 1, FB - 1)], c);
