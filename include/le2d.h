@@ -46,6 +46,14 @@ extern "C" {
 
 const char *le2d_last_error(void);
 
+/* The flux limiter this library was built with (make LIMITER=...): the
+ * reference chooses it at compile time too (simple/le_core.c:36-43, `#define
+ * limiter limiter_superbee`), and superbee is its choice and the default. */
+#define LE2D_LIMITER_SUPERBEE 0
+#define LE2D_LIMITER_MINMOD 1
+#define LE2D_LIMITER_CIR 2
+int le2d_limiter(void);
+
 /* Device memory.  Every device allocation the library makes (state buffers,
  * batch and server arenas, tile and material tables, reduction partials)
  * goes through `alloc`, and is returned through `release` of the allocator

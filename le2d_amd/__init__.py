@@ -126,25 +126,35 @@ _BATCH = {
     "le2d_server_times": ([_VP, ctypes.c_longlong, _DP, _DP, _DP], ctypes.c_int),
     "le2d_server_get_stats": ([_VP, ctypes.POINTER(le2d_server_stats)], ctypes.c_int),
     "le2d_server_destroy": ([_VP], None),
+    "le2d_limiter": ([], ctypes.c_int),
 }
 
 _libs = {}
 
+# include/le2d.h LE2D_LIMITER_*: the flux limiter a library was built with.
+# superbee (the reference's choice) is in LIBDIR; `make limiters` builds the
+# minmod and CIR libraries into lib_minmod/ and lib_cir/.
+LIMITERS = {"superbee": 0, "minmod": 1, "cir": 2}
 
-def lib_path(dtype=np.float64):
+
+def lib_path(dtype=np.float64, limiter="superbee"):
+    if limiter not in LIMITERS:
+        raise ValueError(f"limiter {limiter!r}: superbee, minmod or cir")
     name = "liblecore_d.so" if np.dtype(dtype) == np.float64 else "liblecore_f.so"
-    return os.path.join(LIBDIR, name)
+    base = LIBDIR if limiter == "superbee" else os.path.join(HERE, "lib_" + limiter)
+    return os.path.join(base, name)
 
 
-def load(dtype=np.float64):
-    """Load (once) the C-ABI library for `dtype` and bind its entry points."""
+def load(dtype=np.float64, limiter="superbee"):
+    """Load (once) the C-ABI library for `dtype` built with `limiter` and bind
+    its entry points."""
     dtype = np.dtype(dtype)
-    if dtype in _libs:
-        return _libs[dtype]
-    path = lib_path(dtype)
+    if (dtype, limiter) in _libs:
+        return _libs[(dtype, limiter)]
+    path = lib_path(dtype, limiter)
     if not os.path.exists(path):
         raise RuntimeError(f"le2d_amd: {path} is not built; run __graft_entry__.build() "
-                           "(make -C le2d_amd/csrc)")
+                           "(make -C le2d_amd/csrc all limiters)")
     lib = ctypes.CDLL(path)
     abi = ABI_D if dtype == np.float64 else ABI_F
     abi.bind(lib)
@@ -157,8 +167,10 @@ def load(dtype=np.float64):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
+    if lib.le2d_limiter() != LIMITERS[limiter]:
+        raise RuntimeError(f"le2d_amd: {path} was not built with the {limiter} limiter")
     lib.abi = abi
-    _libs[dtype] = lib
+    _libs[(dtype, limiter)] = lib
     return lib
 
 
@@ -210,9 +222,9 @@ class Task:
     it current."""
 
     def __init__(self, nx, ny, dt, h=(1.0, 1.0), mat=(2.0, 1.0, 1.0), stype=ST_AOS,
-                 dtype=np.float64):
+                 dtype=np.float64, limiter="superbee"):
         self.dtype = np.dtype(dtype)
-        self.lib = load(self.dtype)
+        self.lib = load(self.dtype, limiter)
         self.abi = self.lib.abi
         self.mat = self.abi.le_material()
         self.lib.le_init_material(mat[0], mat[1], mat[2], ctypes.byref(self.mat))

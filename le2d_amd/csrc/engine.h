@@ -12,6 +12,17 @@
 #include <cstdint>
 #include <vector>
 
+// Flux limiter of the TVD scheme, a build-time choice as in the reference
+// (simple/le_core.c:36-43, `#define limiter limiter_superbee`): 0 superbee
+// (the reference's choice and the default), 1 minmod, 2 CIR (no limiting:
+// first-order upwind).  The Makefile's LIMITER=superbee|minmod|cir sets it.
+#ifndef LE2D_LIMITER
+#define LE2D_LIMITER 0
+#endif
+#if LE2D_LIMITER < 0 || LE2D_LIMITER > 2
+#error "LE2D_LIMITER must be 0 (superbee), 1 (minmod) or 2 (cir)"
+#endif
+
 namespace le2d {
 
 // y-edge handling.  CLAMP: positions past the last row read the last row

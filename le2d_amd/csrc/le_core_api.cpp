@@ -843,6 +843,8 @@ int le2d_diff_norms_noalloc(const le2d_view *a, const le2d_view *ref, double out
                                   ref->pitch, ref->plane, a->nx, a->ny, out);
 }
 
+int le2d_limiter(void) { return LE2D_LIMITER; }
+
 int le2d_copy_probe(size_t bytes, int iters, double *ms) {
   return le2d::copy_probe(bytes, iters, ms);
 }

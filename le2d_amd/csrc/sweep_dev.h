@@ -59,12 +59,31 @@ __device__ __forceinline__ T superbee(T r) {
   }
 }
 
+// The build's flux limiter (LE2D_LIMITER).  minmod is the reference's
+// ternary form max(0, min(1, r)) (simple/le_core.c:36), so NaN gives 1 and
+// -0 stays -0 in both types; CIR is 0 (:37).  Both are options beside the
+// default superbee.
+template <typename T>
+__device__ __forceinline__ T limiter(T r) {
+#if LE2D_LIMITER == 1
+  const T m = T(1) > r ? r : T(1);
+  return T(0) > m ? T(0) : m;
+#elif LE2D_LIMITER == 2
+  (void)r;
+  return T(0);
+#else
+  return superbee(r);
+#endif
+}
+
 // Limited flux through the face between upwind node u0 and downwind node u1;
 // um is the node upwind of u0 (simple/le_core.c:50-66 with eps = 1e-6).
 // The reference forms ((phi / 2) * (1 - k)) * (u1 - u0); phi / 2 is exact for
-// every value phi takes (0, 1, 2, r in (1,2), 2r in (0,1)), so
-// (phi / 2) * (1 - k) rounds the same real number as phi * hk with the exact
-// constant hk = (1 - k) / 2, one multiply fewer per face.
+// every value phi takes (superbee: 0, 1, 2, r in (1,2), 2r in (0,1); minmod:
+// 0, 1, r in [0,1); CIR: 0; r is a normal number: its numerator is 0 or at
+// least about 2^-73 in magnitude), so (phi / 2) * (1 - k) rounds the same
+// real number as phi * hk with the exact constant hk = (1 - k) / 2, one
+// multiply fewer per face.
 // Slope ratio.  FP64: IEEE division, as the reference.  FP32 tier: the
 // hardware reciprocal (about 1 ulp) times the numerator; that tier is held to
 // an error bound against FP64, not to bit parity, and the correctly rounded
@@ -99,9 +118,9 @@ __device__ __forceinline__ T face_flux(T um, T u0, T u1, T hk) {
   const T du = u1 - u0;
   const T r = slope_ratio((u0 - um) + eps, du + eps);
   if constexpr (sizeof(T) == 4)
-    return __builtin_fmaf(superbee(r) * hk, du, u0);  // FP32 tier: fused
+    return __builtin_fmaf(limiter(r) * hk, du, u0);  // FP32 tier: fused
   else
-    return u0 + (superbee(r) * hk) * du;
+    return u0 + (limiter(r) * hk) * du;
 }
 
 // Increment of one invariant from its outflow and inflow face fluxes

@@ -27,7 +27,11 @@
  * reduction to orc_run with one material or identical ones, by node-wise
  * equality with the homogeneous sweep in each node's own material, and by
  * the normal-incidence reflection and transmission coefficients of P and S
- * pulses at an interface.
+ * pulses at an interface.  The minmod and CIR builds (ORC_LIMITER, below;
+ * tests/test_limiters.py) are pinned by the limiters' definitions, the
+ * binomial spreading of a pulse under first-order upwind (CIR), minmod's
+ * reduction to Lax-Wendroff and Beam-Warming on concave and convex data, and
+ * the total-variation bound of every limiter.
  */
 #include <math.h>
 #include <stdio.h>
@@ -89,6 +93,27 @@ real_t orc_superbee(real_t r)
 	return orc_max(0.0, orc_max(orc_min(1.0, 2.0 * r), orc_min(2.0, r)));
 }
 
+/* The limiter is a build-time choice, as in the reference (simple/le_core.c:
+ * 36-43, `#define limiter limiter_superbee`): ORC_LIMITER 0 superbee (the
+ * reference's and the default), 1 minmod max(0, min(1, r)) (:36), 2 CIR, the
+ * constant 0.0 (:37). */
+#ifndef ORC_LIMITER
+#define ORC_LIMITER 0
+#endif
+int orc_limiter_kind(void) { return ORC_LIMITER; }
+
+real_t orc_limiter(real_t r)
+{
+#if ORC_LIMITER == 1
+	return orc_max(0.0, orc_min(1.0, r));
+#elif ORC_LIMITER == 2
+	(void)r;
+	return 0.0;
+#else
+	return orc_superbee(r);
+#endif
+}
+
 /* Second-order TVD update of one value, simple/le_core.c:48-69.
  * a = u[i-2], b = u[i-1], u = u[i], d = u[i+1] along the upwind direction. */
 real_t orc_tvd2(real_t c, real_t a, real_t b, real_t u, real_t d)
@@ -107,8 +132,8 @@ real_t orc_tvd2(real_t c, real_t a, real_t b, real_t u, real_t d)
 	den += 1e-6;
 	rm = num / den;
 
-	fp = u + orc_superbee(r) / 2.0 * (1.0 - c) * (d - u);
-	fm = b + orc_superbee(rm) / 2.0 * (1.0 - c) * (u - b);
+	fp = u + orc_limiter(r) / 2.0 * (1.0 - c) * (d - u);
+	fm = b + orc_limiter(rm) / 2.0 * (1.0 - c) * (u - b);
 	return u - c * (fp - fm);
 }
 

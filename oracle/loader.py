@@ -25,10 +25,18 @@ class Oracle:
     """Numpy front end of le2d_ref.c.  Grids are numpy arrays shaped (ny, nx, 5)
     (AOS, node (i, j) at [j, i]) or (5, ny, nx) (SOA planes)."""
 
-    def __init__(self, dtype=np.float64):
+    LIMITERS = {"superbee": 0, "minmod": 1, "cir": 2}
+
+    def __init__(self, dtype=np.float64, limiter="superbee"):
+        """`limiter`: the build's flux limiter (the reference chooses it at
+        compile time, simple/le_core.c:36-43): superbee, minmod or cir."""
         self.dtype = np.dtype(dtype)
         self.real = ctypes.c_double if self.dtype == np.float64 else ctypes.c_float
-        name = "liboracle_d.so" if self.dtype == np.float64 else "liboracle_f.so"
+        if limiter not in self.LIMITERS:
+            raise ValueError(f"limiter {limiter!r}: superbee, minmod or cir")
+        self.limiter_name = limiter
+        stem = "liboracle" if limiter == "superbee" else "liboracle_" + limiter
+        name = stem + ("_d.so" if self.dtype == np.float64 else "_f.so")
         # LE2D_ORACLE_LIBDIR: another build of the oracle (oracle/_san, the
         # sanitizer build of `make san`)
         path = os.path.join(os.environ.get("LE2D_ORACLE_LIBDIR") or HERE, name)
@@ -43,6 +51,10 @@ class Oracle:
         L.orc_material_init.argtypes = [R, R, R, P(self.Mat)]
         L.orc_superbee.argtypes = [R]
         L.orc_superbee.restype = R
+        L.orc_limiter.argtypes = [R]
+        L.orc_limiter.restype = R
+        if L.orc_limiter_kind() != self.LIMITERS[limiter]:
+            raise RuntimeError(f"{path} was not built with the {limiter} limiter")
         L.orc_tvd2.argtypes = [R] * 5
         L.orc_tvd2.restype = R
         grid_args = [P(R), ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -68,6 +80,10 @@ class Oracle:
 
     def superbee(self, r):
         return self.lib.orc_superbee(r)
+
+    def limiter(self, r):
+        """The build's limiter (superbee, minmod or CIR) at r."""
+        return self.lib.orc_limiter(r)
 
     def tvd2(self, c, a, b, u, d):
         return self.lib.orc_tvd2(c, a, b, u, d)

@@ -3,7 +3,8 @@ built from a few exact values: +-0, +-1e-6, 2e-6, 0.5, +-1.  Differences of
 these hit the limiter's edge cases exactly -- slopes of exactly -eps, so
 zero numerators, ratios of +-0 and infinite ratios -- which random inputs
 never reach.  np.array_equal would treat -0 and +0 as equal, so the uint64
-views are compared."""
+views are compared.  Each limiter build (superbee, minmod, CIR) against the
+oracle built with the same limiter."""
 import numpy as np
 import pytest
 
@@ -37,7 +38,8 @@ def _same_bits(a, b):
 
 @pytest.mark.parametrize("stresses", [False, True])
 @pytest.mark.parametrize("pairs", [0, 16])
-def test_special_values_bit_identical(stresses, pairs):
+@pytest.mark.parametrize("limiter", ["superbee", "minmod", "cir"])
+def test_special_values_bit_identical(stresses, pairs, limiter):
     ny, nx = 96, 300
     g0 = _grid(17 + stresses, ny, nx, stresses)
     if not stresses:
@@ -45,11 +47,11 @@ def test_special_values_bit_identical(stresses, pairs):
         assert _negzero_ratio_faces(g0[..., 0]) > 0
         assert _negzero_ratio_faces(np.swapaxes(g0[..., 1], 0, 1)) > 0
     for steps in (1, 4):
-        with le2d_amd.Task(nx, ny, *ARGS) as t:
+        with le2d_amd.Task(nx, ny, *ARGS, limiter=limiter) as t:
             t.load(g0)
             t.set_step_pairs(pairs)
             t.step_n(steps)
             got = t.grid().copy()
         ref = g0.copy()
-        Oracle(np.float64).run(ref, *ARGS, steps)
+        Oracle(np.float64, limiter).run(ref, *ARGS, steps)
         assert _same_bits(got, ref), (steps, int(np.count_nonzero(got.view(np.uint64) != ref.view(np.uint64))))
