@@ -84,16 +84,12 @@ __global__ __launch_bounds__(128, 3) void k_step_batch_pairs_split(
     const InstDesc<T> *__restrict__ inst, const TileJob *__restrict__ tiles, int step_parity) {
   __shared__ W4<T> sw[128];
   __shared__ W4<T> sf[128];
-#ifndef LE2D_SPLIT_RB
-#define LE2D_SPLIT_RB 1
-#endif
-  __shared__ T ring[2 * LE2D_SPLIT_RB * 5 * 64];
+  __shared__ T ring[2 * kSplitRows * 5 * 64];
   const TileJob tj = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const InstDesc<T> &d = inst[tj.inst];
   const int p = d.par ^ step_parity;
-  march_pair_split<T, sizeof(T) == 4, LE2D_SPLIT_RB>(d.buf[p], d.buf[p ^ 1], d.nx, d.ny, d.pitch,
-                                                     d.plane, tj.strip, tj.j0, tj.j1, d.cx, d.cy,
-                                                     sw, sf, ring);
+  march_pair_split<T, sizeof(T) == 4>(d.buf[p], d.buf[p ^ 1], d.nx, d.ny, d.pitch, d.plane,
+                                      tj.strip, tj.j0, tj.j1, d.cx, d.cy, sw, sf, ring);
 }
 
 // Disk initial condition with the host le_set_ball's arithmetic: the node

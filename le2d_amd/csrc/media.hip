@@ -72,7 +72,7 @@ __global__ __launch_bounds__(128, 3) void k_step2_mat_homog_split(
     const SweepCoef<T> *__restrict__ cyt) {
   __shared__ W4<T> sw[128];
   __shared__ W4<T> sf[128];
-  __shared__ T ring[2 * 5 * 64];
+  __shared__ T ring[2 * kSplitRows * 5 * 64];
   const MatTile tj = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const SweepCoef<T> cx = cxt[tj.mat], cy = cyt[tj.mat];
   march_pair_split<T, sizeof(T) == 4>(src, dst, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cx,

@@ -132,29 +132,21 @@ __global__ __launch_bounds__(FB, 2) void k_step2_xy(const T *__restrict__ src, T
 }
 
 // Two fused steps, the stages on two wavefronts (march_pair_split): 128
-// lanes per workgroup, each wavefront one 64-lane window of one stage.
-#ifndef LE2D_SPLIT_WAVES
-#define LE2D_SPLIT_WAVES 3
-#endif
-#ifndef LE2D_SPLIT_HOLD64
-#define LE2D_SPLIT_HOLD64 0
-#endif
-#ifndef LE2D_SPLIT_RB
-#define LE2D_SPLIT_RB 1
-#endif
+// lanes per workgroup, each wavefront one 64-lane window of one stage; three
+// workgroups' waves per SIMD (four spill in FP64).  HOLD: FP32 keeps each
+// slot's y-invariants in registers.
 template <typename T, bool HOLD>
-__global__ __launch_bounds__(128, LE2D_SPLIT_WAVES) void k_step2_split(
+__global__ __launch_bounds__(128, 3) void k_step2_split(
     const T *__restrict__ src, T *__restrict__ dst, int nx, int ny, long pitch, long plane,
     int seg, int nstrips, SweepCoef<T> cx, SweepCoef<T> cy) {
   __shared__ W4<T> sw[128];
   __shared__ W4<T> sf[128];
-  __shared__ T ring[2 * LE2D_SPLIT_RB * 5 * 64];
+  __shared__ T ring[2 * kSplitRows * 5 * 64];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int strip = tile % nstrips, segi = tile / nstrips;
   const int j0 = segi * seg;
   const int j1 = min(j0 + seg, ny);
-  march_pair_split<T, HOLD, LE2D_SPLIT_RB>(src, dst, nx, ny, pitch, plane, strip, j0, j1, cx, cy,
-                                           sw, sf, ring);
+  march_pair_split<T, HOLD>(src, dst, nx, ny, pitch, plane, strip, j0, j1, cx, cy, sw, sf, ring);
 }
 
 // ---------------------------------------------------------------------------
@@ -203,7 +195,7 @@ static int launch_step2_xy(DevGrid *g, const SweepCoef<T> &cx, const SweepCoef<T
   const T *src = static_cast<const T *>(g->buf[g->cur]);
   T *dst = static_cast<T *>(g->buf[g->cur ^ 1]);
   if (FB == 64 && g->tiles.pair_split)
-    hipLaunchKernelGGL((k_step2_split<T, sizeof(T) == 4 || LE2D_SPLIT_HOLD64>),
+    hipLaunchKernelGGL((k_step2_split<T, sizeof(T) == 4>),
                        dim3((unsigned)nblocks), dim3(128), 0,
                        g->stream, src, dst, g->nx, g->ny, g->pitch, g->plane, seg, nstrips, cx,
                        cy);

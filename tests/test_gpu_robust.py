@@ -233,3 +233,25 @@ def test_admission_passes_a_request_that_does_not_fit():
             t_big, t_small = s.times(big)[1], s.times(small)[1]
             assert (t_small < t_big) == small_first, (max_pass, t_small, t_big)
             assert s.times(a)[2] <= t_big + 1e-3
+
+
+def test_grids_below_three_nodes_a_line_are_refused():
+    """The sweeps need three nodes per line (the reference forms the
+    invariants of nodes 0..2 before its loop, simple/le_core.c:482-486, and
+    reads past the line otherwise): a task, a batch instance or a served
+    request with nx or ny below 3 is refused with LE2D_ERR_ARG, and the
+    object stays usable."""
+    for nx, ny in ((2, 5), (5, 2), (1, 1)):
+        with le2d_amd.Task(nx, ny, 0.2) as t:
+            with pytest.raises(le2d_amd.LE2DError, match=r"^le_step_n: 1 .*>= 3"):
+                t.step_n(2)
+    with le2d_amd.Batch(np.float64) as b:
+        with pytest.raises(le2d_amd.LE2DError, match="^le2d_batch_add: 1 "):
+            b.add(2, 9, 0.2)
+        assert b.add(3, 3, 0.2) == 0
+    with le2d_amd.Server(64 * MB, 4) as s:
+        with pytest.raises(le2d_amd.LE2DError, match="^le2d_server_submit: 1 "):
+            s.submit(9, 2, 10)
+        i = s.submit(9, 9, 4, np.float64, keep=True)
+        s.wait(i)
+        assert s.result(i).shape == (9, 9, 5)
