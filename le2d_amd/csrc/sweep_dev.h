@@ -502,6 +502,34 @@ __device__ __forceinline__ W4<T> lane_next(const W4<T> &v) {
   r.d = lane_next(v.d);
   return r;
 }
+// Lane t-1's value on lane t (lane 0 receives 0): DPP wave_shr:1.
+__device__ __forceinline__ float lane_prev(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+}
+
+// FP32 one-wavefront x-sweep faces with shared slopes.  The limiter's
+// numerator (u0 - um) + eps of a face is, operand for operand, the
+// denominator du + eps of the face upwind of it: the face on the left for
+// the invariants moving to +x (a, c), the face on the right for those moving
+// to -x (b, d).  Lane t forms its face's du and du + eps once from lanes nb.y
+// and nb.z (du = w1 - w0 for a, c and w0 - w1 for b, d), and takes the
+// numerators from lanes t-1 and t+1 by lane shifts: two subtractions per
+// face and invariant instead of four, the same operations on the same
+// operands.  Lanes 0 and 63 receive shifted zeros; no output reads their
+// faces.
+__device__ __forceinline__ W4<float> faces_shared(const W4<float> &w0, const W4<float> &w1,
+                                                  const SweepCoef<float> &c) {
+  const float eps = 1e-6f;
+  const float da = w1.a - w0.a, db = w0.b - w1.b, dc = w1.c - w0.c, dd = w0.d - w1.d;
+  const float sa = da + eps, sb = db + eps, sc = dc + eps, sd = dd + eps;
+  W4<float> f;
+  f.a = __builtin_fmaf(limiter(slope_ratio(lane_prev(sa), sa)) * c.hk1, da, w0.a);
+  f.b = __builtin_fmaf(limiter(slope_ratio(lane_next(sb), sb)) * c.hk1, db, w1.b);
+  f.c = __builtin_fmaf(limiter(slope_ratio(lane_prev(sc), sc)) * c.hk2, dc, w0.c);
+  f.d = __builtin_fmaf(limiter(slope_ratio(lane_next(sd), sd)) * c.hk2, dd, w1.d);
+  return f;
+}
 
 // x-sweep of one window row with clamped neighbour lanes: the face between
 // lanes t-1 and t reads lanes nb = (t-2, t-1, t, t+1), each clamped into the
@@ -522,7 +550,7 @@ __device__ __forceinline__ Node<T> window_xsweep_cl(Node<T> u, W4<T> *sw, W4<T> 
     // 2048^2 3-6%).  FP64 forms faces on lanes 2..FB-2 only (branch-free it
     // spills); the others shift zeros that no apply reads.
     if constexpr (sizeof(T) == 4) {
-      const W4<T> f = faces(sw[nb.x], sw[nb.y], sw[nb.z], sw[nb.w], c);
+      const W4<T> f = faces_shared(sw[nb.y], sw[nb.z], c);
       apply(u.vx, u.vy, u.sxx, u.syy, u.sxy, w, f, lane_next(f), c);
     } else {
       W4<T> f = {T(0), T(0), T(0), T(0)};
