@@ -32,6 +32,21 @@ def test_fullsize_fp64_bit_identical_to_oracle():
         assert gd.same_bits(got, ref)
 
 
+@pytest.mark.parametrize("lanes,split,seg", [(64, 1, 128), (64, 0, 80)])
+def test_fullsize_fp64_two_step_forms_bit_identical_to_oracle(lanes, split, seg):
+    """The launch shapes the tuner picks for this grid (64-lane windows, the
+    stages on two wavefronts or one), four two-step passes at full size."""
+    with _task(np.float64) as t:
+        ref = t.grid().copy()
+        t.set_step_pair_window(lanes)
+        t.set_step_pair_split(split)
+        t.set_step_pairs(seg)
+        t.step_n(8)
+        got = t.grid()
+        Oracle(np.float64).run(ref, 0.25, (1.0, 1.0), MAT, 8)
+        assert gd.same_bits(got, ref)
+
+
 def test_fullsize_field_sums_conserved():
     with _task(np.float64) as t:
         s0 = t.grid().reshape(-1, 5).sum(axis=0)
