@@ -269,3 +269,28 @@ def test_long_trajectory_matches_oracle():
     # the fronts reached every edge
     assert all(np.abs(e).max() > 0 for e in (ref[0], ref[-1], ref[:, 0], ref[:, -1]))
     assert gd.same_bits(got, ref)
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 70001), (70001, 3), (4, 131075)])
+def test_extreme_aspect_ratios_match_oracle(nx, ny):
+    """Grids of a few columns and many rows (the y-sweep's segment count
+    passes the 65535 limit of a launch dimension) and of a few rows: fused
+    steps, two-step passes and single sweeps against the oracle."""
+    rng = np.random.default_rng(nx + ny)
+    g0 = rng.standard_normal((ny, nx, 5))
+    dt, h, mat = 0.2, (1.0, 0.9), (2.0, 1.0, 1.5)
+    ref = g0.copy()
+    OD.run(ref, dt, h, mat, 5)
+    with make(g0, dt, h, mat) as t:
+        t.step_n(5)
+        assert gd.same_bits(t.grid(), ref)
+    rx = g0.copy()
+    OD.sweep_x(rx, dt, h[0], mat)
+    ry = rx.copy()
+    OD.sweep_y(ry, dt, h[1], mat)
+    with make(g0, dt, h, mat) as t:
+        t.set_tiles(y_seg=1)
+        t.step_x()
+        assert gd.same_bits(t.grid(), rx)
+        t.step_y()
+        assert gd.same_bits(t.grid(), ry)
