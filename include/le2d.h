@@ -112,7 +112,9 @@ int le2d_get_step_pair_split(le_task *task, int *on);
  * buffer.  le_step_n does this once per
  * grid shape and process on its first call of 8 or more steps, unless the
  * caller has set a height (le2d_set_step_pairs), a material plane is set or
- * the environment has LE2D_TUNE=0. */
+ * the environment has LE2D_TUNE=0.  A task with a material plane keeps the
+ * default window and height; with 64-lane windows its first such call times
+ * the one- and two-wavefront forms instead and keeps the faster. */
 int le2d_tune_step_pairs(le_task *task, int *seg);
 /* Run `passes` two-step passes, timing each launch with HIP events; mean ms
  * of one pass (two steps).  Needs seg > 0 and an ST_AOS task. */
