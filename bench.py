@@ -87,14 +87,21 @@ def time_runs(t, steps, count):
 def sustained(t, n, steps, el, seconds):
     """Rate over at least `seconds` of back-to-back steps (an even count),
     run right after the timed steps: the clock the device holds under
-    sustained load rather than in a short window."""
+    sustained load rather than in a short window.  The state keeps evolving
+    (the wave spreads over more of the grid), so the shader clock and
+    package power are sampled during the window."""
     k = max(steps, int(np.ceil(seconds / max(el / steps, 1e-6))))
     k += k & 1
-    t0 = time.perf_counter()
-    t.step_n(k)
-    t.wait()
-    dt = time.perf_counter() - t0
-    return {"steps": k, "seconds": round(dt, 3), "Gnode_per_s": round(n * n * k / dt / 1e9, 3)}
+
+    def run():
+        t0 = time.perf_counter()
+        t.step_n(k)
+        t.wait()
+        return time.perf_counter() - t0
+    dt, sensors = sampled(run)
+    out = {"steps": k, "seconds": round(dt, 3), "Gnode_per_s": round(n * n * k / dt / 1e9, 3)}
+    out.update(sensors)
+    return out
 
 
 def gpu_sensors():
@@ -775,25 +782,30 @@ def main():
                            "pair_split": t2.get_step_pair_split()},
                  "fp32_vs_fp64_after_steps": args.warmup + args.steps + 4 * args.profile_steps,
                  "fp32_vs_fp64": errs}
-        # the further timed runs come after the comparison, which stays at
-        # the step count above
-        other["repeats"] = repeat_stats(n, args.steps,
-                                        els2 + time_runs(t2, args.steps, args.repeats - 1))
-        other["sustained"] = sustained(t2, n, args.steps, el2, args.sustain)
-        shape2 = (t2.get_step_pairs(), t2.get_step_pair_window(), t2.get_step_pair_split())
-        t2.close()
-        if not args.no_dense:
-            log(f"{n}x{n} {o_name} dense field")
-            other["dense_field"] = dense_report(le2d_amd, n, o_dt, shape2)
+    # The headline type's further timed runs and its sustained window follow
+    # its first run directly (the other type's first run, which the error
+    # comparison above needs at the same step count, is the only work in
+    # between); then the other type's, then the dense-field legs.
     els += time_runs(t, args.steps, args.repeats - 1)
     log(f"{n}x{n} {args.dtype}: sustained {args.sustain:g} s")
     sus = sustained(t, n, args.steps, el, args.sustain)
     shape = (t.get_step_pairs(), t.get_step_pair_window(), t.get_step_pair_split())
     t.close()
+    if other is not None:
+        other["repeats"] = repeat_stats(n, args.steps,
+                                        els2 + time_runs(t2, args.steps, args.repeats - 1))
+        other["sustained"] = sustained(t2, n, args.steps, el2, args.sustain)
+        shape2 = (t2.get_step_pairs(), t2.get_step_pair_window(), t2.get_step_pair_split())
+        t2.close()
     dense = None
     if not args.no_dense:
         log(f"{n}x{n} {args.dtype} dense field")
         dense = dense_report(le2d_amd, n, main_dt, shape)
+        if other is not None:
+            log(f"{n}x{n} {other['dtype']} dense field")
+            other["dense_field"] = dense_report(le2d_amd, n,
+                                                np.float32 if other["dtype"] == "f32"
+                                                else np.float64, shape2)
     copy_bytes = n * n * 5 * es
     copy_gbs = 2 * copy_bytes / le2d_amd.copy_probe(copy_bytes, 10) / 1e6
 
