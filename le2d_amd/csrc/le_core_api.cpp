@@ -509,6 +509,11 @@ int le_save_task(le_task *t, const char *file) {
   fprintf(fp, "LOOKUP_TABLE v_table\n");
   const long nx = t->n.x, ny = t->n.y;
   unsigned char *buf = static_cast<unsigned char *>(malloc((size_t)nx * 4));
+  if (!buf) {
+    fprintf(stderr, "le_save_task: cannot allocate a row buffer of %ld bytes\n", nx * 4);
+    fclose(fp);
+    return 1;
+  }
   for (long j = 0; j < ny; j++) {
     for (long i = 0; i < nx; i++) {
       real vx, vy;
