@@ -410,9 +410,9 @@ class Batch:
     """Many independent grids advanced together (include/le2d.h, le2d_batch_*).
     State arrays are AOS numpy arrays shaped (ny, nx, 5) of the batch dtype."""
 
-    def __init__(self, dtype=np.float32):
+    def __init__(self, dtype=np.float32, limiter="superbee"):
         self.dtype = np.dtype(dtype)
-        self.lib = load(np.float64)  # the batch API is in both libraries
+        self.lib = load(np.float64, limiter)  # the batch API is in both libraries
         self.h = self.lib.le2d_batch_create(self.dtype.itemsize)
         if not self.h:
             raise LE2DError("le2d_batch_create: " + self.lib.le2d_last_error().decode())
@@ -521,8 +521,8 @@ class Server:
     """Serving loop (include/le2d.h, le2d_server_*): requests admitted by HBM
     budget and slots, continuous batching of their steps on the device."""
 
-    def __init__(self, budget_bytes, max_live=256, plain=False):
-        self.lib = load(np.float64)
+    def __init__(self, budget_bytes, max_live=256, plain=False, limiter="superbee"):
+        self.lib = load(np.float64, limiter)
         self.h = self.lib.le2d_server_create(int(budget_bytes), max_live, int(plain))
         if not self.h:
             raise LE2DError("le2d_server_create: " + self.lib.le2d_last_error().decode())

@@ -2,7 +2,8 @@
 lib_cir/) against the oracle built with the same limiter (simple/le_core.c:
 36-43 chooses it at compile time): FP64 bit for bit through every step path
 (single fused steps, two-step passes on each window form, single sweeps, the
-SOA ring edge), FP32 within the tier's bound of the FP64 oracle."""
+SOA ring edge, batches, material planes and served requests), FP32 within the
+tier's bound of the FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -97,3 +98,67 @@ def test_fp32_build_within_tier_bound(limiter):
     Oracle(np.float64, limiter).run(ref, *ARGS, 20)
     rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     assert rel < 1e-5, rel
+
+
+@pytest.mark.parametrize("limiter", LIMITERS)
+def test_batch_matches_oracle(limiter):
+    """The batched kernels of the limiter's library, both two-step forms and
+    the single-step launch, against the oracle per instance."""
+    sizes = [(3, 3), (64, 64), (257, 129), (130, 5), (40, 300)]
+    o = Oracle(np.float64, limiter)
+    for shape in [(64, 16, "pairs_split"), (128, 9, "pairs"), (256, 32, "tiles")]:
+        with le2d_amd.Batch(np.float64, limiter=limiter) as b:
+            states = []
+            for k, (nx, ny) in enumerate(sizes):
+                b.add(nx, ny, *ARGS)
+            b.commit()
+            for k, (nx, ny) in enumerate(sizes):
+                g = _field(40 + k, nx, ny)
+                b.load(k, g)
+                states.append(g)
+            b.set_shape(*shape)
+            b.step(5)
+            for k, g in enumerate(states):
+                ref = g.copy()
+                o.run(ref, *ARGS, 5)
+                assert gd.same_bits(b.read(k), ref), (shape, sizes[k])
+
+
+@pytest.mark.parametrize("limiter", LIMITERS)
+def test_material_plane_matches_oracle(limiter):
+    """Single-material and interface tiles of a two-material plane, one- and
+    two-wavefront two-step passes, against the oracle's heterogeneous scheme
+    built with the same limiter."""
+    nx, ny = 150, 97
+    tab = [(2.0, 1.0, 1.0), (3.1, 1.6, 2.4)]
+    ids = np.zeros((ny, nx), dtype=np.uint8)
+    ids[:, nx // 2:] = 1
+    g0 = _field(50, nx, ny)
+    dt, h = 0.45 / 3.1, (1.0, 1.0)
+    ref = g0.copy()
+    Oracle(np.float64, limiter).run_mat(ref, dt, h, tab, ids, 6)
+    for split in (0, 1):
+        with le2d_amd.Task(nx, ny, dt, h, (2.0, 1.0, 1.0), limiter=limiter) as t:
+            t.load(g0)
+            t.set_step_pair_window(64)
+            t.set_step_pair_split(split)
+            t.set_materials(tab, ids)
+            t.step_n(6)
+            assert gd.same_bits(t.grid(), ref), split
+
+
+@pytest.mark.parametrize("limiter", LIMITERS)
+def test_served_requests_match_oracle(limiter):
+    """Requests of the serving loop in the limiter's library against the
+    oracle: a disk of radius 10 on a zero field, as the request defines."""
+    o = Oracle(np.float64, limiter)
+    with le2d_amd.Server(256 << 20, 8, limiter=limiter) as s:
+        ids = [s.submit(nx, ny, steps, np.float64, dt=0.2, keep=True)
+               for nx, ny, steps in [(70, 50, 9), (33, 120, 6)]]
+        for i, (nx, ny, steps) in zip(ids, [(70, 50, 9), (33, 120, 6)]):
+            s.wait(i)
+            got = s.result(i)
+            ref = np.zeros((ny, nx, 5))
+            o.set_ball(ref, (1.0, 1.0), (nx / 2, ny / 2), 10.0, 1.0)
+            o.run(ref, 0.2, (1.0, 1.0), (2.0, 1.0, 1.0), steps)
+            assert gd.same_bits(got, ref), (nx, ny, steps)
