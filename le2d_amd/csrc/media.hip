@@ -80,7 +80,7 @@ __global__ __launch_bounds__(128, 3) void k_step2_mat_homog_split(
 }
 
 template <typename T, int FB>
-__global__ __launch_bounds__(FB) void k_step2_mat_mixed(const T *__restrict__ src,
+__global__ __launch_bounds__(FB, 3) void k_step2_mat_mixed(const T *__restrict__ src,
                                                        T *__restrict__ dst, int nx, int ny,
                                                        long pitch, long plane,
                                                        const MatTile *__restrict__ tiles,

@@ -221,6 +221,20 @@ struct Node {
   T vx, vy, sxx, sxy, syy;
 };
 
+// c ? a : b field by field.  A conditional on two Node lvalues is copied
+// through a selected address, which keeps an array holding either operand
+// in scratch memory; selecting the values keeps it in registers.
+template <typename T>
+__device__ __forceinline__ Node<T> pick_node(bool c, const Node<T> &a, const Node<T> &b) {
+  Node<T> n;
+  n.vx = c ? a.vx : b.vx;
+  n.vy = c ? a.vy : b.vy;
+  n.sxx = c ? a.sxx : b.sxx;
+  n.sxy = c ? a.sxy : b.sxy;
+  n.syy = c ? a.syy : b.syy;
+  return n;
+}
+
 template <typename T>
 __device__ __forceinline__ Node<T> load_node(const T *__restrict__ p, long plane) {
   Node<T> n;
@@ -1021,12 +1035,12 @@ __device__ __forceinline__ void march_pair_mixed(const T *__restrict__ src, T *_
   // stage 2: Q[k] = x-swept time-(n+1) row at position r-2+k, clamped
 #pragma unroll
   for (int k = 0; k < 4; ++k) Q[k] = next();  // positions j0-2 .. j0+1
-  if (j0 + 1 >= ny) Q[3] = Q[2];
-  if (j0 - 1 < 0) Q[1] = Q[2];
-  if (j0 - 2 < 0) Q[0] = Q[1];
+  Q[3] = pick_node(j0 + 1 >= ny, Q[2], Q[3]);
+  Q[1] = pick_node(j0 - 1 < 0, Q[2], Q[1]);
+  Q[0] = pick_node(j0 - 2 < 0, Q[1], Q[0]);
   for (int r = j0; r < j1; ++r) {
     const Node<T> xn = next();  // position r+2
-    Q[4] = r + 2 >= ny ? Q[3] : xn;
+    Q[4] = pick_node(r + 2 >= ny, Q[3], xn);
     const Node<T> u = yupdate(Q, r);
     if (writer) store_row_node(dst + (long)r * pitch, plane, woff, u);
 #pragma unroll
