@@ -46,16 +46,16 @@ __device__ __forceinline__ T superbee(T r) {
   } else {
     // FP64: max(c, min(r, 2)) is the reference's value bit for bit for every
     // r (NaN and infinities included: the clamp sends NaN to 0, min(NaN, 2)
-    // is 2) except r = -0, where the clamp gives +0 and the reference's
-    // ternaries keep -0.  For r = +-0 the result is +0 (low word 0), so
-    // taking the high word from r there restores the sign: one select of a
-    // 32-bit half instead of a 64-bit select.
+    // is 2) except r = -0, where this gives +0 and the reference's ternaries
+    // keep -0.  No result bit depends on that sign.  The limiter value
+    // reaches the node only through the face flux u + (phi*hk)*du, where a
+    // zero phi changes at most the sign of a zero flux, and through the
+    // increment (u - k*(f_out - f_in)) - u, which is the same for fluxes that
+    // differ only in the sign of a zero: f_out - f_in is then the same nonzero
+    // value or a zero, and (u - (+-0)) - u is +0 for every finite u (NaN for
+    // infinite u either way).
     const double c = __builtin_fmin(__builtin_fmax(r + r, 0.0), 1.0);
-    const double m = __builtin_fmax(c, __builtin_fmin(r, 2.0));
-    const unsigned long long mb = __builtin_bit_cast(unsigned long long, m);
-    const unsigned rh = (unsigned)(__builtin_bit_cast(unsigned long long, r) >> 32);
-    const unsigned hi = r == 0.0 ? rh : (unsigned)(mb >> 32);
-    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | (mb & 0xffffffffull));
+    return __builtin_fmax(c, __builtin_fmin(r, 2.0));
   }
 }
 
