@@ -2,8 +2,10 @@
 built from a few exact values: +-0, +-1e-6, 2e-6, 0.5, +-1.  Differences of
 these hit the limiter's edge cases exactly -- slopes of exactly -eps, so
 zero numerators, ratios of +-0 and infinite ratios -- which random inputs
-never reach.  np.array_equal would treat -0 and +0 as equal, so the uint64
-views are compared.  Each limiter build (superbee, minmod, CIR) against the
+never reach.  The FP64 superbee gives +0 for a ratio of -0, where the
+reference's ternaries give -0; these grids reach such ratios, and the states
+must still agree bit for bit.  np.array_equal would treat -0 and +0 as
+equal, so the uint64 views are compared.  Each limiter build (superbee, minmod, CIR) against the
 oracle built with the same limiter; then magnitudes at both ends of the FP64
 range, and non-finite inputs."""
 import numpy as np
