@@ -79,8 +79,13 @@ __global__ __launch_bounds__(128, 3) void k_step2_mat_homog_split(
                                       cy, sw, sf, ring);
 }
 
+// Interface tiles of the two-step pass, launched beside the single-material
+// kernel.  Beside the two-wavefront form (64 lanes: three waves of at most 168
+// VGPRs per SIMD) a wave of at most 168 VGPRs starts as soon as one
+// neighbouring wave ends; 256-lane windows run beside the one-wavefront form
+// (two waves of 252 VGPRs), where two waves per SIMD lose nothing.
 template <typename T, int FB>
-__global__ __launch_bounds__(FB, 3) void k_step2_mat_mixed(const T *__restrict__ src,
+__global__ __launch_bounds__(FB, FB == 256 ? 2 : 3) void k_step2_mat_mixed(const T *__restrict__ src,
                                                        T *__restrict__ dst, int nx, int ny,
                                                        long pitch, long plane,
                                                        const MatTile *__restrict__ tiles,
