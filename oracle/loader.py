@@ -89,30 +89,37 @@ class Oracle:
         return self.lib.orc_tvd2(c, a, b, u, d)
 
     # --- grid operations (in place) ----------------------------------------
-    def _grid(self, g):
-        assert g.dtype == self.dtype and g.flags.c_contiguous
-        if g.shape[-1] == 5 and g.ndim == 3:
-            return g.shape[1], g.shape[0], 0
-        assert g.ndim == 3 and g.shape[0] == 5
-        return g.shape[2], g.shape[1], 1
+    def _grid(self, g, soa=None):
+        """(nx, ny, soa) of an AOS (ny, nx, 5) or SOA (5, ny, nx) grid.  A
+        (5, n, 5) array is either an AOS grid 5 rows high or an SOA grid 5
+        nodes wide; without `soa` it is taken as AOS, so SOA grids of width 5
+        need soa=True."""
+        assert g.dtype == self.dtype and g.flags.c_contiguous and g.ndim == 3
+        if soa is None:
+            soa = g.shape[-1] != 5
+        if soa:
+            assert g.shape[0] == 5, g.shape
+            return g.shape[2], g.shape[1], 1
+        assert g.shape[-1] == 5, g.shape
+        return g.shape[1], g.shape[0], 0
 
     def _mat(self, mat):
         return mat if isinstance(mat, self.Mat) else self.material(*mat)
 
-    def sweep_x(self, g, dt, hx, mat):
-        nx, ny, soa = self._grid(g)
+    def sweep_x(self, g, dt, hx, mat, soa=None):
+        nx, ny, soa = self._grid(g, soa)
         rc = self.lib.orc_sweep_x(_ptr(g, self.real), nx, ny, soa, dt, hx,
                                   ctypes.byref(self._mat(mat)))
         assert rc == 0
 
-    def sweep_y(self, g, dt, hy, mat, edge=EDGE_CLAMP):
-        nx, ny, soa = self._grid(g)
+    def sweep_y(self, g, dt, hy, mat, edge=EDGE_CLAMP, soa=None):
+        nx, ny, soa = self._grid(g, soa)
         rc = self.lib.orc_sweep_y(_ptr(g, self.real), nx, ny, soa, dt, hy,
                                   ctypes.byref(self._mat(mat)), edge)
         assert rc == 0
 
-    def run(self, g, dt, h, mat, steps):
-        nx, ny, soa = self._grid(g)
+    def run(self, g, dt, h, mat, steps, soa=None):
+        nx, ny, soa = self._grid(g, soa)
         rc = self.lib.orc_run(_ptr(g, self.real), nx, ny, soa, dt, h[0], h[1],
                               ctypes.byref(self._mat(mat)), steps)
         assert rc == 0
@@ -124,19 +131,19 @@ class Oracle:
         assert ids.shape == (ny, nx) and int(ids.max()) < len(tab)
         return tab, ids
 
-    def run_mat(self, g, dt, h, c1c2rho, ids, steps, edge=EDGE_CLAMP):
+    def run_mat(self, g, dt, h, c1c2rho, ids, steps, edge=EDGE_CLAMP, soa=None):
         """Heterogeneous scheme: ids (ny, nx) uint8 material indices into the
         (count, 3) table of (c1, c2, rho)."""
-        nx, ny, soa = self._grid(g)
+        nx, ny, soa = self._grid(g, soa)
         tab, ids = self._table(c1c2rho, ids, nx, ny)
         for _ in range(steps):
             rc = self.lib.orc_step_mat(_ptr(g, self.real), nx, ny, soa, dt, h[0], h[1], len(tab),
                                        tab.ctypes.data, ids.ctypes.data, edge)
             assert rc == 0
 
-    def sweep_mat(self, g, dt, h, c1c2rho, ids, axis, edge=EDGE_CLAMP):
+    def sweep_mat(self, g, dt, h, c1c2rho, ids, axis, edge=EDGE_CLAMP, soa=None):
         """One heterogeneous sweep along x (axis 0, spacing h) or y (axis 1)."""
-        nx, ny, soa = self._grid(g)
+        nx, ny, soa = self._grid(g, soa)
         tab, ids = self._table(c1c2rho, ids, nx, ny)
         rc = self.lib.orc_sweep_mat(_ptr(g, self.real), nx, ny, soa, dt, h, len(tab),
                                     tab.ctypes.data, ids.ctypes.data, axis, edge)
@@ -168,13 +175,13 @@ class Oracle:
         dens = self.energy_density(g.reshape(-1, 5), tab[:, 2], tab[:, 0], tab[:, 1])
         return float(dens.sum()) * h[0] * h[1]
 
-    def set_ball(self, g, h, c, r, s):
-        nx, ny, soa = self._grid(g)
+    def set_ball(self, g, h, c, r, s, soa=None):
+        nx, ny, soa = self._grid(g, soa)
         return self.lib.orc_set_ball(_ptr(g, self.real), nx, ny, soa, h[0], h[1], c[0], c[1],
                                      r, s)
 
-    def save_vtk(self, g, h, path):
-        nx, ny, soa = self._grid(g)
+    def save_vtk(self, g, h, path, soa=None):
+        nx, ny, soa = self._grid(g, soa)
         return self.lib.orc_save_vtk(_ptr(g, self.real), nx, ny, soa, h[0], h[1],
                                      path.encode() if isinstance(path, str) else path)
 

@@ -102,7 +102,7 @@ def test_heterogeneous_ring_edge_soa():
     ids = _ids(nx, ny, "midline", rng)
     g0 = rng.standard_normal((ny, nx, 5))
     ref = np.ascontiguousarray(np.moveaxis(g0, -1, 0))
-    OD.run_mat(ref, 0.12, (1.0, 1.0), tab, ids, 2, edge=EDGE_RING)
+    OD.run_mat(ref, 0.12, (1.0, 1.0), tab, ids, 2, edge=EDGE_RING, soa=True)
     with _task(np.ascontiguousarray(np.moveaxis(g0, -1, 0)), 0.12, (1.0, 1.0), (2.0, 1.0, 1.0),
                ST_SOA) as t:
         t.set_materials(tab, ids)

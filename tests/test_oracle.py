@@ -46,8 +46,8 @@ def test_tiny_golden(dtype):
                 assert np.array_equal(g, data[f"{k}_step{s + 1}"]), f"{k} step {s + 1}"
         p = np.ascontiguousarray(np.moveaxis(g0, -1, 0))
         for s in range(10):
-            o.sweep_x(p, dt, h[0], mat)
-            o.sweep_y(p, dt, h[1], mat, edge=EDGE_RING)
+            o.sweep_x(p, dt, h[0], mat, soa=True)
+            o.sweep_y(p, dt, h[1], mat, edge=EDGE_RING, soa=True)
         assert np.array_equal(np.moveaxis(p, 0, -1), data[k + "_soa10"]), k + " soa"
 
 
@@ -296,10 +296,10 @@ def test_mat_one_material_is_the_reference_scheme(dtype, nx, ny):
         init = np.ascontiguousarray(np.moveaxis(g0, -1, 0)) if soa else g0
         edge = EDGE_RING if soa else EDGE_CLAMP
         ref = init.copy()
-        o.run(ref, dt, h, mat, 4)
+        o.run(ref, dt, h, mat, 4, soa=soa)
         for tab, ids in (([mat], np.zeros((ny, nx), np.uint8)), ([mat] * 3, ids_rand)):
             got = init.copy()
-            o.run_mat(got, dt, h, tab, ids, 4, edge=edge)
+            o.run_mat(got, dt, h, tab, ids, 4, edge=edge, soa=soa)
             assert gd.same_bits(got, ref), (soa, len(tab))
 
 
