@@ -27,6 +27,7 @@ import ctypes
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -288,12 +289,27 @@ def host_boundary_report(le2d_amd, n=8192, steps=100, trips=3):
         t.step()
         t.sync_host()
     el2 = time.perf_counter() - t1
+    # le_save_task after a step: the download and the legacy VTK of |v|
+    saves = []
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "state.vtk")
+        for _ in range(2):
+            t.step()
+            t.wait()
+            t2 = time.perf_counter()
+            if t.save(path) != 0:
+                raise RuntimeError("le_save_task failed")
+            saves.append(time.perf_counter() - t2)
+            vtk_bytes = os.path.getsize(path)
     t.close()
     return {"workload": f"{n}x{n} FP64 through le_step on the host mirror",
             "driver_pattern": {"steps": steps, "seconds": round(el, 3),
                                "Gnode_per_s": round(n * n * steps / el / 1e9, 3)},
             "round_trip_per_step": {"ms_per_step": round(el2 / trips * 1e3, 2),
-                                    "Gnode_per_s": round(n * n * trips / el2 / 1e9, 3)}}
+                                    "Gnode_per_s": round(n * n * trips / el2 / 1e9, 3)},
+            "save_task": {"seconds": round(min(saves), 3), "vtk_bytes": vtk_bytes,
+                          "what": "le_save_task after a step: download of the state and "
+                                  "the VTK of |v| written to a local file (best of 2)"}}
 
 
 def slab_report(le2d_amd, n=8192, parts=8, steps=400):

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <unordered_map>
 #include <vector>
@@ -488,6 +489,34 @@ void le_set_ball(le_task *t, const le_vec2 c, const real r, const real s) {
 
 // simple/le_core.c:99-169: header lines, then |v| as big-endian float32,
 // rows j outer, nodes i inner.
+// |v| of rows [j0, j1) as big-endian floats (simple/le_core.c:99-169: the
+// speed sqrt(vx*vx + vy*vy) formed in real, rounded to float, written
+// byte-swapped, row by row).
+static void vtk_rows(const le_task *t, long j0, long j1, unsigned char *out) {
+  const long nx = t->n.x, ny = t->n.y;
+  for (long j = j0; j < j1; j++) {
+    for (long i = 0; i < nx; i++) {
+      real vx, vy;
+      if (t->stype == ST_AOS) {
+        const le_node *nd = reinterpret_cast<const le_node *>(t->grid) + (i + j * nx);
+        vx = nd->v.x;
+        vy = nd->v.y;
+      } else {
+        vx = t->grid[i + j * nx];
+        vy = t->grid[nx * ny + i + j * nx];
+      }
+      const float v = (float)::sqrt((double)(vx * vx + vy * vy));
+      uint32_t bits;
+      memcpy(&bits, &v, 4);
+      out[0] = (unsigned char)(bits >> 24);
+      out[1] = (unsigned char)(bits >> 16);
+      out[2] = (unsigned char)(bits >> 8);
+      out[3] = (unsigned char)bits;
+      out += 4;
+    }
+  }
+}
+
 int le_save_task(le_task *t, const char *file) {
   TaskState *st = find(t);
   int rc = sync_host(t, st);
@@ -508,32 +537,35 @@ int le_save_task(le_task *t, const char *file) {
   fprintf(fp, "SCALARS v float 1\n");
   fprintf(fp, "LOOKUP_TABLE v_table\n");
   const long nx = t->n.x, ny = t->n.y;
-  unsigned char *buf = static_cast<unsigned char *>(malloc((size_t)nx * 4));
+  const long row_bytes = nx * 4;
+  // rows in chunks of about 64 MB, each chunk converted by up to 16 threads
+  // and written with one fwrite
+  const long chunk = std::min(ny, std::max(1L, (64L << 20) / row_bytes));
+  unsigned char *buf = static_cast<unsigned char *>(malloc((size_t)(chunk * row_bytes)));
   if (!buf) {
-    fprintf(stderr, "le_save_task: cannot allocate a row buffer of %ld bytes\n", nx * 4);
+    fprintf(stderr, "le_save_task: cannot allocate a buffer of %ld bytes\n", chunk * row_bytes);
     fclose(fp);
     return 1;
   }
-  for (long j = 0; j < ny; j++) {
-    for (long i = 0; i < nx; i++) {
-      real vx, vy;
-      if (t->stype == ST_AOS) {
-        const le_node *nd = reinterpret_cast<const le_node *>(t->grid) + (i + j * nx);
-        vx = nd->v.x;
-        vy = nd->v.y;
-      } else {
-        vx = t->grid[i + j * nx];
-        vy = t->grid[nx * ny + i + j * nx];
+  const long threads = std::max(1L, std::min(16L, (long)std::thread::hardware_concurrency()));
+  for (long j0 = 0; j0 < ny; j0 += chunk) {
+    const long j1 = std::min(ny, j0 + chunk), rows = j1 - j0;
+    const long parts = rows * nx < (1L << 18) ? 1 : std::min(threads, rows);
+    std::vector<std::thread> pool;
+    // part k: rows [rows*k/parts, rows*(k+1)/parts) of the chunk; part 0 on
+    // the caller's thread
+    for (long k = 1; k < parts; ++k) {
+      const long a = rows * k / parts, b = rows * (k + 1) / parts;
+      try {
+        pool.emplace_back(vtk_rows, t, j0 + a, j0 + b, buf + a * row_bytes);
+      } catch (...) {  // no thread: the caller's thread converts the rest
+        vtk_rows(t, j0 + a, j1, buf + a * row_bytes);
+        break;
       }
-      const float v = (float)::sqrt((double)(vx * vx + vy * vy));
-      uint32_t bits;
-      memcpy(&bits, &v, 4);
-      buf[4 * i + 0] = (unsigned char)(bits >> 24);
-      buf[4 * i + 1] = (unsigned char)(bits >> 16);
-      buf[4 * i + 2] = (unsigned char)(bits >> 8);
-      buf[4 * i + 3] = (unsigned char)bits;
     }
-    fwrite(buf, 1, (size_t)nx * 4, fp);
+    vtk_rows(t, j0, j0 + rows / parts, buf);
+    for (std::thread &th : pool) th.join();
+    fwrite(buf, 1, (size_t)(rows * row_bytes), fp);
   }
   free(buf);
   fclose(fp);
