@@ -487,8 +487,6 @@ void le_set_ball(le_task *t, const le_vec2 c, const real r, const real s) {
   st->host_dirty = true;
 }
 
-// simple/le_core.c:99-169: header lines, then |v| as big-endian float32,
-// rows j outer, nodes i inner.
 // |v| of rows [j0, j1) as big-endian floats (simple/le_core.c:99-169: the
 // speed sqrt(vx*vx + vy*vy) formed in real, rounded to float, written
 // byte-swapped, row by row).
@@ -517,6 +515,8 @@ static void vtk_rows(const le_task *t, long j0, long j1, unsigned char *out) {
   }
 }
 
+// simple/le_core.c:99-169: header lines, then |v| as big-endian float32,
+// rows j outer, nodes i inner.
 int le_save_task(le_task *t, const char *file) {
   TaskState *st = find(t);
   int rc = sync_host(t, st);
