@@ -80,19 +80,15 @@ __global__ __launch_bounds__(128, 3) void k_step2_mat_homog_split(
 }
 
 // Interface tiles of the two-step pass, launched beside the single-material
-// kernel.  Beside the two-wavefront form (64 lanes: three waves of at most 168
-// VGPRs per SIMD) a wave of at most 168 VGPRs starts as soon as one
-// neighbouring wave ends; 256-lane windows run beside the one-wavefront form
-// (two waves of 252 VGPRs), where two waves per SIMD lose nothing.
+// kernel.  At 64 lanes that is the two-wavefront form, three waves of at most
+// 168 VGPRs per SIMD: held to the same bound, a wave of this kernel starts as
+// soon as one neighbouring wave ends (128 lanes keep the bound too, without
+// spilling).  At 256 lanes two waves per SIMD keep it free of spills.
 template <typename T, int FB>
-__global__ __launch_bounds__(FB, FB == 256 ? 2 : 3) void k_step2_mat_mixed(const T *__restrict__ src,
-                                                       T *__restrict__ dst, int nx, int ny,
-                                                       long pitch, long plane,
-                                                       const MatTile *__restrict__ tiles,
-                                                       const unsigned char *__restrict__ ids,
-                                                       long idp,
-                                                       const SweepCoef<T> *__restrict__ cxt,
-                                                       const SweepCoef<T> *__restrict__ cyt) {
+__global__ __launch_bounds__(FB, FB == 256 ? 2 : 3) void k_step2_mat_mixed(
+    const T *__restrict__ src, T *__restrict__ dst, int nx, int ny, long pitch, long plane,
+    const MatTile *__restrict__ tiles, const unsigned char *__restrict__ ids, long idp,
+    const SweepCoef<T> *__restrict__ cxt, const SweepCoef<T> *__restrict__ cyt) {
   __shared__ Node<T> sn[FB];
   const MatTile tj = tiles[blockIdx.x];
   march_pair_mixed<T, FB>(src, dst, ids, idp, nx, ny, pitch, plane, tj.strip, tj.j0, tj.j1, cxt,
