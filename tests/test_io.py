@@ -1,4 +1,5 @@
-"""Raw dump/restore (CPU: the host mirror round-trips bit for bit) and the
+"""le_save_task of a large grid (CPU: byte for byte the oracle writer's file),
+raw dump/restore (CPU: the host mirror round-trips bit for bit) and the
 command-line drivers (GPU: the reference's own simple/main.c linked to this
 library, and le2d_main.c, reproduce the reference's result.vtk)."""
 import hashlib
@@ -13,6 +14,29 @@ import golden_data as gd
 import le2d_amd
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.parametrize("dtype,stype", [(np.float64, le2d_amd.ST_AOS),
+                                         (np.float32, le2d_amd.ST_SOA)])
+def test_save_task_of_a_large_grid_is_the_oracle_writers_file(tmp_path, dtype, stype):
+    """4100 x 4200 nodes: le_save_task converts the |v| rows on several
+    threads, in two chunks of about 64 MB; the file is the oracle writer's
+    byte for byte (CPU: a task never stepped saves its host mirror)."""
+    from oracle import Oracle
+    nx, ny, h = 4100, 4200, (0.7, 1.3)
+    rng = np.random.default_rng(3)
+    with le2d_amd.Task(nx, ny, 0.2, h, dtype=dtype, stype=stype) as t:
+        v = t.view()
+        if stype == le2d_amd.ST_AOS:
+            v[..., :2] = rng.standard_normal((ny, nx, 2))
+        else:
+            v[:2] = rng.standard_normal((2, ny, nx))
+        assert t.save(str(tmp_path / "a.vtk")) == 0
+        g = np.ascontiguousarray(v.copy())
+    assert Oracle(dtype).save_vtk(g, h, str(tmp_path / "b.vtk"),
+                                  soa=stype == le2d_amd.ST_SOA) == 0
+    a, b = (tmp_path / "a.vtk").read_bytes(), (tmp_path / "b.vtk").read_bytes()
+    assert len(a) > (64 << 20) and a == b
 
 
 @pytest.mark.parametrize("dtype,stype", [(np.float64, le2d_amd.ST_AOS),
