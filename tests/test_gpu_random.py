@@ -5,8 +5,8 @@ values (slopes of exactly -eps: zero, infinite and 0/0 ratios), each case
 through every step path at a random launch shape: paired le_step calls,
 le_step_n's two-step passes (random segment height, window width and, at 64
 lanes, one or two wavefronts), the single x- and y-sweeps and the SOA ring
-edge; then batches of random instances in every launch form, and random
-material planes.  FP64 bit for bit (uint64 views); FP32 on the Gaussian
+edge; then batches of random instances in every launch form, random
+material planes, and the minmod and CIR builds.  FP64 bit for bit (uint64 views); FP32 on the Gaussian
 fields within the tier's 1e-5 relative L2 of the FP64 oracle."""
 import numpy as np
 import pytest
@@ -172,3 +172,37 @@ def test_random_material_planes_fp64():
                     t.set_step_pair_split(shape[2])
                     t.step_n(steps)
                 assert _same(t.grid(), ref), (seed, nm, seed % 3, steps, shape, paired_calls)
+
+
+@pytest.mark.parametrize("limiter", ["minmod", "cir"])
+def test_random_cases_other_limiters(limiter):
+    """The minmod and CIR libraries on 16 of the random cases (paired
+    le_step calls, le_step_n at the case's launch shape, the SOA ring edge)
+    against the oracle built with the same limiter, bit for bit."""
+    o = Oracle(np.float64, limiter)
+    for seed in range(16):
+        nx, ny, dt, h, mat, g0, steps, shape, _ = _case(seed)
+        ref = g0.copy()
+        o.run(ref, dt, h, mat, steps)
+        with le2d_amd.Task(nx, ny, dt, h, mat, limiter=limiter) as t:
+            t.load(g0)
+            for _ in range(steps):
+                t.step()
+            assert _same(t.grid(), ref), (limiter, seed, "le_step", steps)
+        with le2d_amd.Task(nx, ny, dt, h, mat, limiter=limiter) as t:
+            t.load(g0)
+            t.set_step_pairs(shape[0])
+            t.set_step_pair_window(shape[1])
+            t.set_step_pair_split(shape[2])
+            t.step_n(steps)
+            assert _same(t.grid(), ref), (limiter, seed, "le_step_n", steps, shape)
+        p0 = np.ascontiguousarray(np.moveaxis(g0, -1, 0))
+        rs = p0.copy()
+        for _ in range(steps):
+            o.sweep_x(rs, dt, h[0], mat, soa=True)
+            o.sweep_y(rs, dt, h[1], mat, edge=EDGE_RING, soa=True)
+        with le2d_amd.Task(nx, ny, dt, h, mat, stype=ST_SOA, limiter=limiter) as t:
+            t.load(p0)
+            for _ in range(steps):
+                t.step()
+            assert _same(t.grid(), rs), (limiter, seed, "le_step_soa", steps)
